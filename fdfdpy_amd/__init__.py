@@ -1,0 +1,7 @@
+"""MI355X-native drop-in for fdfdpy's 2D FDFD solve path.
+
+``from fdfdpy_amd import Simulation`` mirrors ``from fdfdpy import Simulation``
+(fdfdpy/__init__.py:3).
+"""
+
+name = "fdfdpy_amd"

@@ -1,0 +1,1110 @@
+// fdfd_hip.hip -- gfx950 kernels and C ABI for the 2D FDFD solve path.
+//
+// Operator: the SC-PML 5-point stencil of fdfdpy/linalg.py:41-107 stored as
+// five complex planes. Factor: a block sweep over x (blocks are y-columns of
+// size n = ny, padded to TB) of the x-non-periodic part T of A:
+//     S_0 = T_0,  S_k = T_k - diag(l_k) G_{k-1} diag(u_{k-1}),  G_k = S_k^-1
+// with l_k / u_k the couplings to the previous / next column. Each G_k is
+// formed by tiled Gauss-Jordan (TB x TB pivot tiles inverted with partial
+// pivoting inside the tile, rank-TB updates on MFMA). The periodic x-wrap of
+// A (derivatives.py:22,25) is left to refinement against the exact operator.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+#include <cmath>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "../../include/fdfd_hip.h"
+
+namespace {
+
+constexpr int TB = 32;        // pivot tile / output tile edge
+constexpr int TBP = TB + 1;   // padded LDS row
+constexpr int NT = 256;       // threads per workgroup (4 waves)
+constexpr int MAXJ = 8;       // jobs per batched step launch
+constexpr int MAXR = 4;       // right-hand sides per sweep pass
+constexpr int NORM_PARTS = 512;
+
+// constants.py:3-6
+constexpr double EPSILON_0 = 8.85418782e-12;
+constexpr double MU_0 = 1.25663706e-6;
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(FDFD_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));       \
+  } while (0)
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+inline int pad_n(int n) { return (n + TB - 1) / TB * TB; }
+
+// ---------------------------------------------------------------- complex
+struct cd { double r, i; };
+__host__ __device__ inline cd cmul(cd a, cd b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__host__ __device__ inline cd cadd(cd a, cd b) { return {a.r + b.r, a.i + b.i}; }
+__host__ __device__ inline cd csub(cd a, cd b) { return {a.r - b.r, a.i - b.i}; }
+__host__ __device__ inline cd cscale(cd a, double s) { return {a.r * s, a.i * s}; }
+// 1/z by Smith's method (no overflow of |z|^2)
+template <typename T>
+__host__ __device__ inline void crecip(T zr, T zi, T &outr, T &outi) {
+  if (fabs(zr) >= fabs(zi)) {
+    T t = zi / zr, den = zr + zi * t;
+    outr = T(1) / den;
+    outi = -t / den;
+  } else {
+    T t = zr / zi, den = zi + zr * t;
+    outr = t / den;
+    outi = T(-1) / den;
+  }
+}
+__host__ __device__ inline cd cinv(cd z) { cd o; crecip<double>(z.r, z.i, o.r, o.i); return o; }
+
+// ------------------------------------------------------- PML (pml.py:7-39)
+// s = 1 - i sigma(l)/(omega eps0 L0), sigma = sigma_max (l/dw)^4,
+// sigma_max = -(m+1) lnR / (2 eta0 dw), m = 4, lnR = -12.
+__device__ inline cd s_of_l(double l, double dw, double omega, double L0) {
+  const double eta0 = sqrt(MU_0 / EPSILON_0);
+  const double sig_max = -(4.0 + 1.0) * (-12.0) / (2.0 * eta0 * dw);
+  double q = l / dw;
+  double sig = sig_max * (q * q) * (q * q);
+  return {1.0, -sig / (omega * EPSILON_0 * L0)};
+}
+// kind 0 = 'f', 1 = 'b'; left layer i <= npml, right layer i > nw - npml.
+__device__ inline cd sfactor(int kind, int i, int nw, int npml, double hw, double omega, double L0) {
+  if (npml < 1) return {1.0, 0.0};
+  double dw = npml * hw;
+  if (i <= npml) {
+    double l = kind == 0 ? hw * (npml - i + 0.5) : hw * (double)(npml - i + 1);
+    return s_of_l(l, dw, omega, L0);
+  }
+  if (i > nw - npml) {
+    double l = kind == 0 ? hw * (i - (nw - npml) - 0.5) : hw * (double)(i - (nw - npml) - 1);
+    return s_of_l(l, dw, omega, L0);
+  }
+  return {1.0, 0.0};
+}
+
+__device__ inline cd ld_c(const double *p, size_t idx) { return {p[2 * idx], p[2 * idx + 1]}; }
+__device__ inline void st_c(double *p, size_t idx, cd v) { p[2 * idx] = v.r; p[2 * idx + 1] = v.i; }
+
+// ------------------------------------------ assembly (linalg.py:41-107)
+__global__ void __launch_bounds__(NT) k_assemble(fdfd_grid g, const double *__restrict__ eps,
+                                                 double *__restrict__ op) {
+  const int j = blockIdx.x * NT + threadIdx.x;
+  const int i = blockIdx.y;
+  const int nx = g.nx, ny = g.ny;
+  if (j >= ny) return;
+  // cell sizes as dL() forms them: (N*dl)/N (linalg.py:25-33, simulation.py:33-34)
+  const double hx = ((double)nx * g.dl) / nx, hy = ((double)ny * g.dl) / ny;
+  const int ip = (i + 1) % nx, im = (i + nx - 1) % nx;
+  const int jp = (j + 1) % ny, jm = (j + ny - 1) % ny;
+  cd isxf = cinv(sfactor(0, i, nx, g.npml_x, hx, g.omega, g.L0));
+  cd isxb = cinv(sfactor(1, i, nx, g.npml_x, hx, g.omega, g.L0));
+  cd isxb_p = cinv(sfactor(1, ip, nx, g.npml_x, hx, g.omega, g.L0));
+  cd isyf = cinv(sfactor(0, j, ny, g.npml_y, hy, g.omega, g.L0));
+  cd isyb = cinv(sfactor(1, j, ny, g.npml_y, hy, g.omega, g.L0));
+  cd isyb_p = cinv(sfactor(1, jp, ny, g.npml_y, hy, g.omega, g.L0));
+  const double eps0_ = EPSILON_0 * g.L0, mu0_ = MU_0 * g.L0;
+  const size_t M = (size_t)nx * ny, p = (size_t)i * ny + j;
+  // stretched difference rows: Dxf row i = (1/sxf_i)/hx * (x[i+1]-x[i]),
+  // Dxb row i = (1/sxb_i)/hx * (x[i]-x[i-1])
+  cd fx = cscale(isxf, 1.0 / hx), fy = cscale(isyf, 1.0 / hy);
+  cd bx = cscale(isxb, 1.0 / hx), bxp = cscale(isxb_p, 1.0 / hx);
+  cd by = cscale(isyb, 1.0 / hy), byp = cscale(isyb_p, 1.0 / hy);
+  cd xm, xp, ym, yp, cc;
+  cd e = ld_c(eps, p);
+  if (g.pol == FDFD_POL_EZ) {
+    fx = cscale(fx, 1.0 / mu0_);
+    fy = cscale(fy, 1.0 / mu0_);
+    xp = cmul(fx, bxp);
+    xm = cmul(fx, bx);
+    yp = cmul(fy, byp);
+    ym = cmul(fy, by);
+    cd w2e = cscale(cscale(e, eps0_), g.omega * g.omega);
+    cc = cadd(csub(csub(csub(csub({0, 0}, xp), xm), yp), ym), w2e);
+  } else {
+    // grid_average(eps0' eps, 'x') at (i,j) and (i+1,j); 'y' at (i,j), (i,j+1)
+    cd e_im = ld_c(eps, (size_t)im * ny + j), e_ip = ld_c(eps, (size_t)ip * ny + j);
+    cd e_jm = ld_c(eps, (size_t)i * ny + jm), e_jp = ld_c(eps, (size_t)i * ny + jp);
+    cd ee = cscale(e, eps0_);
+    cd ex_i = cscale(cadd(cscale(e_im, eps0_), ee), 0.5);
+    cd ex_ip = cscale(cadd(ee, cscale(e_ip, eps0_)), 0.5);
+    cd ey_j = cscale(cadd(cscale(e_jm, eps0_), ee), 0.5);
+    cd ey_jp = cscale(cadd(ee, cscale(e_jp, eps0_)), 0.5);
+    xp = cmul(fx, cmul(cinv(ex_ip), bxp));
+    xm = cmul(fx, cmul(cinv(ex_i), bx));
+    yp = cmul(fy, cmul(cinv(ey_jp), byp));
+    ym = cmul(fy, cmul(cinv(ey_j), by));
+    cc = cadd(csub(csub(csub(csub({0, 0}, xp), xm), yp), ym), {g.omega * g.omega * mu0_, 0.0});
+  }
+  st_c(op, FDFD_OP_C * M + p, cc);
+  st_c(op, FDFD_OP_XM * M + p, xm);
+  st_c(op, FDFD_OP_XP * M + p, xp);
+  st_c(op, FDFD_OP_YM * M + p, ym);
+  st_c(op, FDFD_OP_YP * M + p, yp);
+}
+
+// ------------------------------------------------- stencil y = A x / b - A x
+__global__ void __launch_bounds__(NT) k_stencil(int nx, int ny, const double *__restrict__ op,
+                                                const double *__restrict__ x,
+                                                const double *b, double *y) {
+  const int j = blockIdx.x * NT + threadIdx.x;
+  const int i = blockIdx.y, r = blockIdx.z;
+  if (j >= ny) return;
+  const size_t M = (size_t)nx * ny, p = (size_t)i * ny + j;
+  const double *xr = x + 2 * M * r;
+  const int ip = (i + 1) % nx, im = (i + nx - 1) % nx;
+  const int jp = (j + 1) % ny, jm = (j + ny - 1) % ny;
+  cd acc = cmul(ld_c(op, FDFD_OP_C * M + p), ld_c(xr, p));
+  acc = cadd(acc, cmul(ld_c(op, FDFD_OP_XM * M + p), ld_c(xr, (size_t)im * ny + j)));
+  acc = cadd(acc, cmul(ld_c(op, FDFD_OP_XP * M + p), ld_c(xr, (size_t)ip * ny + j)));
+  acc = cadd(acc, cmul(ld_c(op, FDFD_OP_YM * M + p), ld_c(xr, (size_t)i * ny + jm)));
+  acc = cadd(acc, cmul(ld_c(op, FDFD_OP_YP * M + p), ld_c(xr, (size_t)i * ny + jp)));
+  if (b) acc = csub(ld_c(b + 2 * M * r, p), acc);
+  st_c(y + 2 * M * r, p, acc);
+}
+
+// ------------------------------------------------------------------ MFMA
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mac(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // row of accumulator t in lane l (column is l % 16)
+  static __device__ inline int crow(int l, int t) { return (l >> 4) + 4 * t; }
+};
+template <> struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mac(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int crow(int l, int t) { return 4 * (l >> 4) + t; }
+};
+
+// Layout of the 16x16x4 instructions (wave64): lane l supplies A[l%16][l/16]
+// and B[l/16][l%16]; accumulator t of lane l is C[crow(l, t)][l%16], with
+// crow = l/16 + 4t for f64 and 4(l/16) + t for f32.
+__global__ void k_selftest_mfma_f64(const double *a, const double *b, double *d, int raw) {
+  const int l = threadIdx.x;
+  Mfma<double>::v4 c = {0, 0, 0, 0};
+  c = Mfma<double>::mac(a[(l % 16) * 4 + l / 16], b[(l / 16) * 16 + l % 16], c);
+  for (int t = 0; t < 4; ++t) {
+    if (raw) d[l * 4 + t] = c[t];
+    else d[Mfma<double>::crow(l, t) * 16 + l % 16] = c[t];
+  }
+}
+__global__ void k_selftest_mfma_f32(const double *a, const double *b, double *d) {
+  const int l = threadIdx.x;
+  Mfma<float>::v4 c = {0, 0, 0, 0};
+  c = Mfma<float>::mac((float)a[(l % 16) * 4 + l / 16], (float)b[(l / 16) * 16 + l % 16], c);
+  for (int t = 0; t < 4; ++t) d[Mfma<float>::crow(l, t) * 16 + l % 16] = c[t];
+}
+
+// --------------------------------------------------------- factor: step job
+template <typename T> struct FJob {
+  const double *op;  // band storage of this operator
+  T *G;              // [m][2][np][np] planar (re plane, im plane) inverses
+  T *W;              // [2][np][np] ping-pong partner of G_k
+  T *Pinv;           // [2 slots][2][TB][TB]
+  int *status;       // set to 1 on a non-finite / zero pivot
+  int m, n, np, nt;  // blocks, block size, padded block size, tiles per edge
+  int k, p;          // step performed by this launch; k < 0: initial pivot
+  int h, ngroups;    // tile rows per workgroup, row groups
+};
+template <typename T> struct FBatch {
+  int njobs;
+  int wg_start[MAXJ + 1];
+  FJob<T> job[MAXJ];
+};
+
+// Element (a, b) of the working matrix of block k before step p. For p == 0
+// this is S_k = T_k - l_k G_{k-1} u_{k-1} (identity on the padding).
+template <typename T>
+__device__ inline void x_elem(const FJob<T> &J, int k, int p, const T *src, int a, int b,
+                              T &re, T &im) {
+  const size_t np2 = (size_t)J.np * J.np;
+  if (p > 0) {
+    re = src[(size_t)a * J.np + b];
+    im = src[np2 + (size_t)a * J.np + b];
+    return;
+  }
+  const int n = J.n;
+  if (a >= n || b >= n) {
+    re = (a == b) ? T(1) : T(0);
+    im = T(0);
+    return;
+  }
+  const size_t M = (size_t)J.m * n;
+  const size_t pa = (size_t)k * n + a;
+  double vr = 0, vi = 0;
+  if (a == b) { cd c = ld_c(J.op, FDFD_OP_C * M + pa); vr += c.r; vi += c.i; }
+  if (b == (a + 1) % n) { cd c = ld_c(J.op, FDFD_OP_YP * M + pa); vr += c.r; vi += c.i; }
+  if (b == (a + n - 1) % n) { cd c = ld_c(J.op, FDFD_OP_YM * M + pa); vr += c.r; vi += c.i; }
+  if (k > 0) {
+    const T *Gp = J.G + (size_t)(k - 1) * 2 * np2;
+    cd gv = {(double)Gp[(size_t)a * J.np + b], (double)Gp[np2 + (size_t)a * J.np + b]};
+    cd l = ld_c(J.op, FDFD_OP_XM * M + pa);
+    cd u = ld_c(J.op, FDFD_OP_XP * M + (size_t)(k - 1) * n + b);
+    cd s = cmul(cmul(l, gv), u);
+    vr -= s.r;
+    vi -= s.i;
+  }
+  re = (T)vr;
+  im = (T)vi;
+}
+
+template <typename T> struct Lds {
+  T Pr[TB][TBP], Pi[TB][TBP];  // pivot inverse
+  T Rr[TB][TBP], Ri[TB][TBP];  // P * X[p,J]
+  T Ar[TB][TBP], Ai[TB][TBP];  // X[I,p], X[p,J] staging, inversion scratch
+  int perm[TB];
+  int piv;
+};
+
+template <typename T>
+__device__ inline void load_tile(const FJob<T> &J, int k, int p, const T *src, int ti, int tj,
+                                 T (*dr)[TBP], T (*di)[TBP]) {
+  for (int e = threadIdx.x; e < TB * TB; e += NT) {
+    const int a = e / TB, b = e % TB;
+    x_elem(J, k, p, src, ti * TB + a, tj * TB + b, dr[a][b], di[a][b]);
+  }
+}
+
+// c += sign * A[quadrant rows, :] * B[:, quadrant cols] (complex, on MFMA)
+template <typename T>
+__device__ inline void cgemm_quad(typename Mfma<T>::v4 &cr, typename Mfma<T>::v4 &ci,
+                                  const T (*Ar)[TBP], const T (*Ai)[TBP], const T (*Br)[TBP],
+                                  const T (*Bi)[TBP], int qi, int qj, int lane, T sign) {
+  const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int kk = 0; kk < TB; kk += 4) {
+    const T ar = sign * Ar[qi * 16 + r][kk + kq], ai = sign * Ai[qi * 16 + r][kk + kq];
+    const T br = Br[kk + kq][qj * 16 + r], bi = Bi[kk + kq][qj * 16 + r];
+    cr = Mfma<T>::mac(ar, br, cr);
+    cr = Mfma<T>::mac(-ai, bi, cr);
+    ci = Mfma<T>::mac(ar, bi, ci);
+    ci = Mfma<T>::mac(ai, br, ci);
+  }
+}
+
+// In-place inverse of the TB x TB tile (Ar, Ai) by Gauss-Jordan with
+// partial pivoting inside the tile; column swaps undone at the end.
+template <typename T>
+__device__ void tile_inverse(Lds<T> &L, int *status) {
+  const int tid = threadIdx.x;
+  bool bad = false;
+  for (int j = 0; j < TB; ++j) {
+    if (tid < 64) {
+      T mag = T(-1);
+      int idx = tid;
+      if (tid >= j && tid < TB) mag = fabs(L.Ar[tid][j]) + fabs(L.Ai[tid][j]);
+      for (int off = 32; off >= 1; off >>= 1) {
+        T om = __shfl_xor(mag, off);
+        int oi = __shfl_xor(idx, off);
+        if (om > mag || (om == mag && oi < idx)) { mag = om; idx = oi; }
+      }
+      if (tid == 0) { L.piv = idx; L.perm[j] = idx; }
+    }
+    __syncthreads();
+    const int piv = L.piv;
+    if (piv != j && tid < TB) {
+      T t = L.Ar[j][tid]; L.Ar[j][tid] = L.Ar[piv][tid]; L.Ar[piv][tid] = t;
+      t = L.Ai[j][tid]; L.Ai[j][tid] = L.Ai[piv][tid]; L.Ai[piv][tid] = t;
+    }
+    __syncthreads();
+    const T pr = L.Ar[j][j], pi = L.Ai[j][j];
+    if (!(fabs(pr) + fabs(pi) > T(0)) || !isfinite(pr) || !isfinite(pi)) bad = true;
+    T dr, di;
+    crecip<T>(pr, pi, dr, di);
+    T nr[TB * TB / NT], ni[TB * TB / NT];
+#pragma unroll
+    for (int q = 0; q < TB * TB / NT; ++q) {
+      const int e = tid + NT * q, i = e / TB, c = e % TB;
+      if (i == j) {
+        if (c == j) { nr[q] = dr; ni[q] = di; }
+        else {
+          const T ar = L.Ar[j][c], ai = L.Ai[j][c];
+          nr[q] = ar * dr - ai * di; ni[q] = ar * di + ai * dr;
+        }
+      } else {
+        const T fr = L.Ar[i][j], fi = L.Ai[i][j];
+        const T fdr = fr * dr - fi * di, fdi = fr * di + fi * dr;  // f * d
+        if (c == j) { nr[q] = -fdr; ni[q] = -fdi; }
+        else {
+          const T ar = L.Ar[j][c], ai = L.Ai[j][c];
+          nr[q] = L.Ar[i][c] - (fdr * ar - fdi * ai);
+          ni[q] = L.Ai[i][c] - (fdr * ai + fdi * ar);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TB * TB / NT; ++q) {
+      const int e = tid + NT * q, i = e / TB, c = e % TB;
+      L.Ar[i][c] = nr[q]; L.Ai[i][c] = ni[q];
+    }
+    __syncthreads();
+  }
+  for (int j = TB - 1; j >= 0; --j) {
+    const int pj = L.perm[j];
+    if (pj != j && tid < TB) {
+      T t = L.Ar[tid][j]; L.Ar[tid][j] = L.Ar[tid][pj]; L.Ar[tid][pj] = t;
+      t = L.Ai[tid][j]; L.Ai[tid][j] = L.Ai[tid][pj]; L.Ai[tid][pj] = t;
+    }
+    __syncthreads();
+  }
+  if (bad && tid == 0) *status = 1;
+}
+
+template <typename T>
+__device__ inline void store_pinv(const Lds<T> &L, T *slot) {
+  for (int e = threadIdx.x; e < TB * TB; e += NT) {
+    const int a = e / TB, b = e % TB;
+    slot[e] = L.Ar[a][b];
+    slot[TB * TB + e] = L.Ai[a][b];
+  }
+}
+
+// S_{k+1}[0,0] tile from G_k[0,0] held in (Ar, Ai): T_{k+1} - l G u.
+template <typename T>
+__device__ inline void schur_next_00(const FJob<T> &J, int k, Lds<T> &L) {
+  const int n = J.n;
+  const size_t M = (size_t)J.m * n;
+  for (int e = threadIdx.x; e < TB * TB; e += NT) {
+    const int a = e / TB, b = e % TB;
+    double vr, vi;
+    if (a >= n || b >= n) { vr = (a == b) ? 1.0 : 0.0; vi = 0.0; }
+    else {
+      const size_t pa = (size_t)(k + 1) * n + a;
+      vr = 0; vi = 0;
+      if (a == b) { cd c = ld_c(J.op, FDFD_OP_C * M + pa); vr += c.r; vi += c.i; }
+      if (b == (a + 1) % n) { cd c = ld_c(J.op, FDFD_OP_YP * M + pa); vr += c.r; vi += c.i; }
+      if (b == (a + n - 1) % n) { cd c = ld_c(J.op, FDFD_OP_YM * M + pa); vr += c.r; vi += c.i; }
+      cd gv = {(double)L.Ar[a][b], (double)L.Ai[a][b]};
+      cd s = cmul(cmul(ld_c(J.op, FDFD_OP_XM * M + pa), gv),
+                  ld_c(J.op, FDFD_OP_XP * M + (size_t)k * n + b));
+      vr -= s.r; vi -= s.i;
+    }
+    L.Ar[a][b] = (T)vr;  // each (a,b) read and written by the same thread
+    L.Ai[a][b] = (T)vi;
+  }
+  __syncthreads();
+}
+
+// One Gauss-Jordan step (k, p) for every job of the batch.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
+  __shared__ Lds<T> L;
+  int jb = 0;
+  while (jb + 1 < batch.njobs && (int)blockIdx.x >= batch.wg_start[jb + 1]) ++jb;
+  const FJob<T> &J = batch.job[jb];
+  const int local = blockIdx.x - batch.wg_start[jb];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int qi = w >> 1, qj = w & 1;
+  const size_t np2 = (size_t)J.np * J.np;
+  const int nt = J.nt;
+
+  if (J.k < 0) {  // initial pivot: inverse of T_0[0,0]
+    if (local == 0) {
+      load_tile(J, 0, 0, (const T *)nullptr, 0, 0, L.Ar, L.Ai);
+      __syncthreads();
+      tile_inverse(L, J.status);
+      store_pinv(L, J.Pinv);  // slot of global step 0
+    }
+    return;
+  }
+  const int k = J.k, p = J.p;
+  const int tj = local % nt, grp = local / nt;
+  const int i0 = grp * J.h, i1 = min(nt, i0 + J.h);
+  T *Gk = J.G + (size_t)k * 2 * np2;
+  const T *src = (p == 0) ? nullptr : (((nt - p) % 2 == 0) ? Gk : J.W);
+  T *dst = ((nt - 1 - p) % 2 == 0) ? Gk : J.W;
+  const long step = (long)k * nt + p;
+  const T *Pslot = J.Pinv + (size_t)(step & 1) * 2 * TB * TB;
+  T *Pnext = J.Pinv + (size_t)((step + 1) & 1) * 2 * TB * TB;
+
+  for (int e = tid; e < TB * TB; e += NT) {
+    L.Pr[e / TB][e % TB] = Pslot[e];
+    L.Pi[e / TB][e % TB] = Pslot[TB * TB + e];
+  }
+  if (tj != p) {
+    load_tile(J, k, p, src, p, tj, L.Ar, L.Ai);  // X[p, J]
+    __syncthreads();
+    typename Mfma<T>::v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    cgemm_quad<T>(cr, ci, L.Pr, L.Pi, L.Ar, L.Ai, qi, qj, lane, T(1));
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+      L.Rr[a][b] = cr[t];
+      L.Ri[a][b] = ci[t];
+    }
+  }
+  __syncthreads();
+
+  for (int ti = i0; ti < i1; ++ti) {
+    typename Mfma<T>::v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    if (ti == p) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+        cr[t] = (tj == p) ? L.Pr[a][b] : L.Rr[a][b];
+        ci[t] = (tj == p) ? L.Pi[a][b] : L.Ri[a][b];
+      }
+    } else {
+      load_tile(J, k, p, src, ti, p, L.Ar, L.Ai);  // X[I, p]
+      if (tj != p) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+          T xr, xi;
+          x_elem(J, k, p, src, ti * TB + a, tj * TB + b, xr, xi);
+          cr[t] = xr;
+          ci[t] = xi;
+        }
+      }
+      __syncthreads();
+      if (tj == p) cgemm_quad<T>(cr, ci, L.Ar, L.Ai, L.Pr, L.Pi, qi, qj, lane, T(-1));
+      else cgemm_quad<T>(cr, ci, L.Ar, L.Ai, L.Rr, L.Ri, qi, qj, lane, T(-1));
+    }
+    // store Y[I, J]
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+      const size_t idx = (size_t)(ti * TB + a) * J.np + tj * TB + b;
+      dst[idx] = cr[t];
+      dst[np2 + idx] = ci[t];
+    }
+    const bool next_pivot = (p + 1 < nt) && ti == p + 1 && tj == p + 1;
+    const bool next_block = (p == nt - 1) && ti == 0 && tj == 0 && k + 1 < J.m;
+    __syncthreads();  // Ar/Ai free again
+    if (next_pivot || next_block) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+        L.Ar[a][b] = cr[t];
+        L.Ai[a][b] = ci[t];
+      }
+      __syncthreads();
+      if (next_block) schur_next_00(J, k, L);
+      tile_inverse(L, J.status);
+      store_pinv(L, Pnext);
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------- sweeps
+// Forward: y_k = G_k (r_k - l_k y_{k-1}); y is [nrhs][nx][np] complex T.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_sweep_fwd(const T *__restrict__ G, const double *__restrict__ op,
+                                                  const double *__restrict__ r, T *__restrict__ y,
+                                                  int nx, int ny, int np, int k, int nrhs) {
+  __shared__ T vr[MAXR][768 + TB], vi[MAXR][768 + TB];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const size_t np2 = (size_t)np * np, M = (size_t)nx * ny;
+  const T *Gk = G + (size_t)k * 2 * np2;
+  const int a = blockIdx.x * 4 + w;
+  for (int r0 = 0; r0 < nrhs; r0 += MAXR) {
+    const int nr = min(MAXR, nrhs - r0);
+    for (int e = tid; e < nr * np; e += NT) {
+      const int rr = e / np, c = e % np;
+      double xr = 0, xi = 0;
+      if (c < ny) {
+        cd v = ld_c(r, (size_t)(r0 + rr) * M + (size_t)k * ny + c);
+        if (k > 0) {
+          const T *yp = y + 2 * (((size_t)(r0 + rr) * nx + (k - 1)) * np + c);
+          v = csub(v, cmul(ld_c(op, FDFD_OP_XM * M + (size_t)k * ny + c), cd{(double)yp[0], (double)yp[1]}));
+        }
+        xr = v.r; xi = v.i;
+      }
+      vr[rr][c] = (T)xr;
+      vi[rr][c] = (T)xi;
+    }
+    __syncthreads();
+    if (a < np) {
+      T ar[MAXR], ai[MAXR];
+#pragma unroll
+      for (int q = 0; q < MAXR; ++q) { ar[q] = 0; ai[q] = 0; }
+      for (int c = lane; c < np; c += 64) {
+        const T gr = Gk[(size_t)a * np + c], gi = Gk[np2 + (size_t)a * np + c];
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) {
+          if (q < nr) {
+            ar[q] += gr * vr[q][c] - gi * vi[q][c];
+            ai[q] += gr * vi[q][c] + gi * vr[q][c];
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < MAXR; ++q) {
+        for (int off = 32; off >= 1; off >>= 1) {
+          ar[q] += __shfl_xor(ar[q], off);
+          ai[q] += __shfl_xor(ai[q], off);
+        }
+      }
+      if (lane == 0) {
+        for (int q = 0; q < nr; ++q) {
+          T *yo = y + 2 * (((size_t)(r0 + q) * nx + k) * np + a);
+          yo[0] = ar[q];
+          yo[1] = ai[q];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Backward: d_k = y_k - G_k (u_k d_{k+1}) in place of y; x_k += d_k (FP64).
+template <typename T>
+__global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const double *__restrict__ op,
+                                                  T *__restrict__ y, double *__restrict__ x,
+                                                  int nx, int ny, int np, int k, int nrhs) {
+  __shared__ T vr[MAXR][768 + TB], vi[MAXR][768 + TB];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const size_t np2 = (size_t)np * np, M = (size_t)nx * ny;
+  const T *Gk = G + (size_t)k * 2 * np2;
+  const int a = blockIdx.x * 4 + w;
+  for (int r0 = 0; r0 < nrhs; r0 += MAXR) {
+    const int nr = min(MAXR, nrhs - r0);
+    for (int e = tid; e < nr * np; e += NT) {
+      const int rr = e / np, c = e % np;
+      double xr = 0, xi = 0;
+      if (c < ny && k + 1 < nx) {
+        const T *dn = y + 2 * (((size_t)(r0 + rr) * nx + (k + 1)) * np + c);
+        cd v = cmul(ld_c(op, FDFD_OP_XP * M + (size_t)k * ny + c), cd{(double)dn[0], (double)dn[1]});
+        xr = v.r; xi = v.i;
+      }
+      vr[rr][c] = (T)xr;
+      vi[rr][c] = (T)xi;
+    }
+    __syncthreads();
+    if (a < np) {
+      T ar[MAXR], ai[MAXR];
+#pragma unroll
+      for (int q = 0; q < MAXR; ++q) { ar[q] = 0; ai[q] = 0; }
+      if (k + 1 < nx) {
+        for (int c = lane; c < np; c += 64) {
+          const T gr = Gk[(size_t)a * np + c], gi = Gk[np2 + (size_t)a * np + c];
+#pragma unroll
+          for (int q = 0; q < MAXR; ++q) {
+            if (q < nr) {
+              ar[q] += gr * vr[q][c] - gi * vi[q][c];
+              ai[q] += gr * vi[q][c] + gi * vr[q][c];
+            }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) {
+          for (int off = 32; off >= 1; off >>= 1) {
+            ar[q] += __shfl_xor(ar[q], off);
+            ai[q] += __shfl_xor(ai[q], off);
+          }
+        }
+      }
+      if (lane == 0) {
+        for (int q = 0; q < nr; ++q) {
+          T *yk = y + 2 * (((size_t)(r0 + q) * nx + k) * np + a);
+          const T dr = yk[0] - ar[q], di = yk[1] - ai[q];
+          yk[0] = dr;
+          yk[1] = di;
+          if (a < ny) {
+            double *xo = x + 2 * ((size_t)(r0 + q) * M + (size_t)k * ny + a);
+            xo[0] += (double)dr;
+            xo[1] += (double)di;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ norms
+// part[r][blk] = sum |w v|^2 over a slice; w = stretch weight when weighted.
+__device__ inline double stretch_abs2(const fdfd_grid &g, int i, int j) {
+  const double hx = ((double)g.nx * g.dl) / g.nx, hy = ((double)g.ny * g.dl) / g.ny;
+  cd s = cmul(sfactor(0, i, g.nx, g.npml_x, hx, g.omega, g.L0),
+              sfactor(0, j, g.ny, g.npml_y, hy, g.omega, g.L0));
+  return s.r * s.r + s.i * s.i;
+}
+__global__ void __launch_bounds__(NT) k_norm_part(fdfd_grid g, const double *__restrict__ v,
+                                                  int weighted, double *__restrict__ part) {
+  __shared__ double red[NT];
+  const size_t M = (size_t)g.nx * g.ny;
+  const int r = blockIdx.y;
+  double acc = 0;
+  for (size_t p = (size_t)blockIdx.x * NT + threadIdx.x; p < M; p += (size_t)gridDim.x * NT) {
+    cd z = ld_c(v + 2 * M * r, p);
+    double a2 = z.r * z.r + z.i * z.i;
+    if (weighted) a2 *= stretch_abs2(g, (int)(p / g.ny), (int)(p % g.ny));
+    acc += a2;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(size_t)r * gridDim.x + blockIdx.x] = red[0];
+}
+__global__ void __launch_bounds__(NT) k_norm_final(const double *__restrict__ part, int nparts,
+                                                   double *__restrict__ out) {
+  __shared__ double red[NT];
+  const int r = blockIdx.x;
+  double acc = 0;
+  for (int q = threadIdx.x; q < nparts; q += NT) acc += part[(size_t)r * nparts + q];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[r] = red[0];
+}
+
+// --------------------------------------------------- elementwise helpers
+__global__ void __launch_bounds__(NT) k_stretch_scale(fdfd_grid g, double *v, int inverse) {
+  const int j = blockIdx.x * NT + threadIdx.x, i = blockIdx.y, r = blockIdx.z;
+  if (j >= g.ny) return;
+  const double hx = ((double)g.nx * g.dl) / g.nx, hy = ((double)g.ny * g.dl) / g.ny;
+  cd s = cmul(sfactor(0, i, g.nx, g.npml_x, hx, g.omega, g.L0),
+              sfactor(0, j, g.ny, g.npml_y, hy, g.omega, g.L0));
+  if (inverse) s = cinv(s);
+  const size_t M = (size_t)g.nx * g.ny, p = (size_t)r * M + (size_t)i * g.ny + j;
+  st_c(v, p, cmul(ld_c(v, p), s));
+}
+
+__global__ void __launch_bounds__(NT) k_grad_eps(fdfd_grid g, const double *__restrict__ lam,
+                                                 const double *__restrict__ e, double *__restrict__ grad) {
+  const size_t M = (size_t)g.nx * g.ny;
+  const size_t p = (size_t)blockIdx.x * NT + threadIdx.x;
+  if (p >= M) return;
+  const double w2e = g.omega * g.omega * EPSILON_0 * g.L0;
+  cd z = cmul(ld_c(lam, p), ld_c(e, p));
+  grad[p] = -w2e * z.r;
+}
+
+__global__ void __launch_bounds__(NT) k_kerr(fdfd_grid g, double *__restrict__ op,
+                                             const double *__restrict__ base_c,
+                                             const double *__restrict__ e,
+                                             const double *__restrict__ chi,
+                                             double *__restrict__ eps_nl) {
+  const size_t M = (size_t)g.nx * g.ny;
+  const size_t p = (size_t)blockIdx.x * NT + threadIdx.x;
+  if (p >= M) return;
+  cd z = ld_c(e, p);
+  const double enl = chi[p] * (z.r * z.r + z.i * z.i);
+  cd c = ld_c(base_c, p);
+  c.r += g.omega * g.omega * EPSILON_0 * g.L0 * enl;
+  st_c(op, FDFD_OP_C * M + p, c);
+  if (eps_nl) eps_nl[p] = enl;
+}
+
+
+// Secondary fields (simulation.py:133-175): Ez -> (Hx, Hy), Hz -> (Ex, Ey).
+__global__ void __launch_bounds__(NT) k_fields(fdfd_grid g, const double *__restrict__ eps,
+                                               const double *__restrict__ x, double *__restrict__ f1,
+                                               double *__restrict__ f2, int averaging) {
+  const int j = blockIdx.x * NT + threadIdx.x, i = blockIdx.y, r = blockIdx.z;
+  if (j >= g.ny) return;
+  const int nx = g.nx, ny = g.ny;
+  const double hx = ((double)nx * g.dl) / nx, hy = ((double)ny * g.dl) / ny;
+  const size_t M = (size_t)nx * ny, p = (size_t)i * ny + j;
+  const double *xr = x + 2 * M * r;
+  const int im = (i + nx - 1) % nx, jm = (j + ny - 1) % ny;
+  cd X = ld_c(xr, p);
+  // Dyb X and Dxb X (derivatives.py:7-34 with the 1/s stretch, pml.py:79-97)
+  cd dyb = cmul(cinv(sfactor(1, j, ny, g.npml_y, hy, g.omega, g.L0)),
+                cscale(csub(X, ld_c(xr, (size_t)i * ny + jm)), 1.0 / hy));
+  cd dxb = cmul(cinv(sfactor(1, i, nx, g.npml_x, hx, g.omega, g.L0)),
+                cscale(csub(X, ld_c(xr, (size_t)im * ny + j)), 1.0 / hx));
+  const cd inv_iw = {0.0, -1.0 / g.omega};  // 1/(i omega)
+  cd a, b;
+  if (g.pol == FDFD_POL_EZ) {
+    const cd c = cscale(inv_iw, 1.0 / (MU_0 * g.L0));
+    a = cscale(cmul(c, dyb), -1.0);
+    b = cmul(c, dxb);
+  } else {
+    const double eps0_ = EPSILON_0 * g.L0;
+    cd ee = cscale(ld_c(eps, p), eps0_), ex = ee, ey = ee;
+    if (averaging) {
+      ex = cscale(cadd(cscale(ld_c(eps, (size_t)im * ny + j), eps0_), ee), 0.5);
+      ey = cscale(cadd(cscale(ld_c(eps, (size_t)i * ny + jm), eps0_), ee), 0.5);
+    }
+    a = cmul(inv_iw, cmul(cinv(ey), dyb));
+    b = cscale(cmul(inv_iw, cmul(cinv(ex), dxb)), -1.0);
+  }
+  st_c(f1 + 2 * M * r, p, a);
+  st_c(f2 + 2 * M * r, p, b);
+}
+
+// ---------------------------------------------------------------- host side
+int check_grid(const fdfd_grid *g) {
+  if (!g) return fail(FDFD_ERR_ARG, "grid is NULL");
+  if (g->nx < 2 || g->ny < 2) return fail(FDFD_ERR_ARG, "grid must be at least 2x2, got %dx%d", g->nx, g->ny);
+  if (g->ny > 768) return fail(FDFD_ERR_ARG, "block size ny=%d exceeds the supported 768 (sweep along the shorter axis)", g->ny);
+  if (g->npml_x < 0 || g->npml_y < 0) return fail(FDFD_ERR_ARG, "both elements of NPML must be >= 0");
+  if (g->pol != FDFD_POL_EZ && g->pol != FDFD_POL_HZ) return fail(FDFD_ERR_ARG, "pol must be Ez or Hz");
+  if (!(g->omega > 0) || !(g->dl > 0) || !(g->L0 > 0)) return fail(FDFD_ERR_ARG, "omega, dl and L0 must be positive");
+  return FDFD_OK;
+}
+int check_prec(int precision) {
+  if (precision != FDFD_PREC_FP64 && precision != FDFD_PREC_C64)
+    return fail(FDFD_ERR_ARG, "precision must be FDFD_PREC_FP64 or FDFD_PREC_C64, got %d", precision);
+  return FDFD_OK;
+}
+inline size_t elem_size(int precision) { return precision == FDFD_PREC_FP64 ? 8 : 4; }
+
+struct FactorWork {
+  void *W; void *Pinv; int *status;
+};
+FactorWork factor_work_layout(const fdfd_grid *g, int precision, void *work) {
+  const size_t es = elem_size(precision);
+  char *base = (char *)work;
+  FactorWork fw;
+  fw.status = (int *)base;
+  size_t off = 256;
+  fw.Pinv = base + off;
+  off += align_up(2 * 2 * TB * TB * es, 256);
+  fw.W = base + off;
+  return fw;
+}
+
+struct SolveWork {
+  double *r; void *y; double *part; double *norms;
+};
+SolveWork solve_work_layout(const fdfd_grid *g, int precision, int nrhs, void *work) {
+  const size_t M = (size_t)g->nx * g->ny, np = pad_n(g->ny), es = elem_size(precision);
+  char *base = (char *)work;
+  SolveWork sw;
+  size_t off = 0;
+  sw.r = (double *)(base + off); off += align_up(16 * M * nrhs, 256);
+  sw.y = base + off;             off += align_up(2 * es * (size_t)g->nx * np * nrhs, 256);
+  sw.part = (double *)(base + off); off += align_up(8 * (size_t)NORM_PARTS * nrhs * 2, 256);
+  sw.norms = (double *)(base + off);
+  return sw;
+}
+
+template <typename T>
+int launch_step(const FBatch<T> &b, hipStream_t s) {
+  const int grid = b.wg_start[b.njobs];
+  hipLaunchKernelGGL(k_gj_step<T>, dim3(grid), dim3(NT), 0, s, b);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+template <typename T>
+int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, hipStream_t s) {
+  const int n = g->ny, np = pad_n(n), nt = np / TB, m = g->nx;
+  FactorWork fw = factor_work_layout(g, sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64, work);
+  HIP_TRY(hipMemsetAsync(fw.status, 0, sizeof(int), s));
+  FBatch<T> b;
+  memset(&b, 0, sizeof b);
+  b.njobs = 1;
+  FJob<T> &J = b.job[0];
+  J.op = op; J.G = (T *)factor; J.W = (T *)fw.W; J.Pinv = (T *)fw.Pinv; J.status = fw.status;
+  J.m = m; J.n = n; J.np = np; J.nt = nt;
+  J.h = 1;
+  J.ngroups = nt;
+  J.k = -1; J.p = 0;
+  b.wg_start[0] = 0; b.wg_start[1] = 1;
+  int rc = launch_step(b, s);
+  if (rc) return rc;
+  b.wg_start[1] = nt * J.ngroups;
+  for (int k = 0; k < m; ++k) {
+    for (int p = 0; p < nt; ++p) {
+      J.k = k; J.p = p;
+      rc = launch_step(b, s);
+      if (rc) return rc;
+    }
+  }
+  return FDFD_OK;
+}
+
+template <typename T>
+int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const double *r,
+               double *x, int nrhs, void *y, hipStream_t s) {
+  const int nx = g->nx, ny = g->ny, np = pad_n(ny);
+  const dim3 grid((np + 3) / 4);
+  for (int k = 0; k < nx; ++k) {
+    hipLaunchKernelGGL(k_sweep_fwd<T>, grid, dim3(NT), 0, s, (const T *)factor, op, r, (T *)y, nx, ny, np, k, nrhs);
+  }
+  for (int k = nx - 1; k >= 0; --k) {
+    hipLaunchKernelGGL(k_sweep_bwd<T>, grid, dim3(NT), 0, s, (const T *)factor, op, (T *)y, x, nx, ny, np, k, nrhs);
+  }
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+int norms_async(const fdfd_grid *g, const double *v, int nrhs, int weighted, double *part,
+                double *out, hipStream_t s) {
+  const size_t M = (size_t)g->nx * g->ny;
+  int parts = (int)std::min<size_t>(NORM_PARTS, (M + NT - 1) / NT);
+  hipLaunchKernelGGL(k_norm_part, dim3(parts, nrhs), dim3(NT), 0, s, *g, v, weighted, part);
+  hipLaunchKernelGGL(k_norm_final, dim3(nrhs), dim3(NT), 0, s, (const double *)part, parts, out);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int fdfd_abi_version(void) { return FDFD_ABI_VERSION; }
+const char *fdfd_last_error(void) { return g_err.c_str(); }
+int fdfd_block_padding(void) { return TB; }
+
+size_t fdfd_op_bytes(const fdfd_grid *g) {
+  if (check_grid(g)) return 0;
+  return (size_t)5 * g->nx * g->ny * 16;
+}
+size_t fdfd_factor_bytes(const fdfd_grid *g, int precision) {
+  if (check_grid(g) || check_prec(precision)) return 0;
+  const size_t np = pad_n(g->ny);
+  return (size_t)g->nx * 2 * np * np * elem_size(precision);
+}
+size_t fdfd_factor_workspace_bytes(const fdfd_grid *g, int precision) {
+  if (check_grid(g) || check_prec(precision)) return 0;
+  const size_t np = pad_n(g->ny), es = elem_size(precision);
+  return 256 + align_up(2 * 2 * TB * TB * es, 256) + align_up(2 * np * np * es, 256);
+}
+size_t fdfd_solve_workspace_bytes(const fdfd_grid *g, int precision, int nrhs) {
+  if (check_grid(g) || check_prec(precision) || nrhs < 1) return 0;
+  const size_t M = (size_t)g->nx * g->ny, np = pad_n(g->ny), es = elem_size(precision);
+  return align_up(16 * M * nrhs, 256) + align_up(2 * es * (size_t)g->nx * np * nrhs, 256) +
+         align_up(8 * (size_t)NORM_PARTS * nrhs * 2, 256) + align_up(8 * 4 * (size_t)nrhs, 256);
+}
+
+int fdfd_assemble(const fdfd_grid *g, const double *eps, double *op, hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!eps || !op) return fail(FDFD_ERR_ARG, "eps and op must be device pointers");
+  hipLaunchKernelGGL(k_assemble, dim3((g->ny + NT - 1) / NT, g->nx), dim3(NT), 0, stream, *g, eps, op);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+int fdfd_apply(const fdfd_grid *g, const double *op, const double *x, double *y, int nrhs,
+               hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!op || !x || !y || nrhs < 1) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_apply");
+  hipLaunchKernelGGL(k_stencil, dim3((g->ny + NT - 1) / NT, g->nx, nrhs), dim3(NT), 0, stream,
+                     g->nx, g->ny, op, x, (const double *)nullptr, y);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+int fdfd_residual(const fdfd_grid *g, const double *op, const double *b, const double *x,
+                  double *r, int nrhs, hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!op || !x || !b || !r || nrhs < 1) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_residual");
+  hipLaunchKernelGGL(k_stencil, dim3((g->ny + NT - 1) / NT, g->nx, nrhs), dim3(NT), 0, stream,
+                     g->nx, g->ny, op, x, b, r);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+int fdfd_factor(const fdfd_grid *g, const double *op, void *factor, int precision, void *work,
+                hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (int rc = check_prec(precision)) return rc;
+  if (!op || !factor || !work) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_factor");
+  return precision == FDFD_PREC_FP64 ? factor_impl<double>(g, op, factor, work, stream)
+                                     : factor_impl<float>(g, op, factor, work, stream);
+}
+
+int fdfd_factor_status(const fdfd_grid *g, int precision, void *work, hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (int rc = check_prec(precision)) return rc;
+  FactorWork fw = factor_work_layout(g, precision, work);
+  int st = 0;
+  HIP_TRY(hipMemcpyAsync(&st, fw.status, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (st) return fail(FDFD_ERR_NUMERIC, "zero or non-finite pivot in the block factorization");
+  return FDFD_OK;
+}
+
+int fdfd_sweep(const fdfd_grid *g, const double *op, const void *factor, int precision,
+               const double *r, double *x, int nrhs, void *work, hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (int rc = check_prec(precision)) return rc;
+  if (!op || !factor || !r || !x || !work || nrhs < 1) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_sweep");
+  SolveWork sw = solve_work_layout(g, precision, nrhs, work);
+  return precision == FDFD_PREC_FP64 ? sweep_impl<double>(g, op, factor, r, x, nrhs, sw.y, stream)
+                                     : sweep_impl<float>(g, op, factor, r, x, nrhs, sw.y, stream);
+}
+
+int fdfd_norms(const fdfd_grid *g, const double *v, int nrhs, void *work, double *out,
+               hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!v || !work || !out || nrhs < 1) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_norms");
+  SolveWork sw = solve_work_layout(g, FDFD_PREC_FP64, nrhs, work);
+  if (int rc = norms_async(g, v, nrhs, 0, sw.part, sw.norms, stream)) return rc;
+  HIP_TRY(hipMemcpyAsync(out, sw.norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  return FDFD_OK;
+}
+
+int fdfd_stretch_scale(const fdfd_grid *g, double *v, int nrhs, int inverse, hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!v || nrhs < 1) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_stretch_scale");
+  hipLaunchKernelGGL(k_stretch_scale, dim3((g->ny + NT - 1) / NT, g->nx, nrhs), dim3(NT), 0, stream,
+                     *g, v, inverse);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int precision,
+               const double *b, double *x, int nrhs, int transpose, double tol, int max_iter,
+               void *work, fdfd_solve_stats *stats, double *rel_res, hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (int rc = check_prec(precision)) return rc;
+  if (!op || !factor || !b || !x || !work || nrhs < 1 || max_iter < 1 || !(tol > 0))
+    return fail(FDFD_ERR_ARG, "bad arguments to fdfd_solve");
+  SolveWork sw = solve_work_layout(g, precision, nrhs, work);
+  const size_t M = (size_t)g->nx * g->ny;
+  fdfd_solve_stats st;
+  memset(&st, 0, sizeof st);
+  std::vector<double> nb(nrhs), nr(nrhs), prev(nrhs, INFINITY);
+  // ||b|| (in the caller's system)
+  if (int rc = norms_async(g, b, nrhs, 0, sw.part, sw.norms, stream)) return rc;
+  HIP_TRY(hipMemcpyAsync(nb.data(), sw.norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipMemsetAsync(x, 0, 16 * M * nrhs, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  bool all_zero = true;
+  for (int q = 0; q < nrhs; ++q) all_zero = all_zero && nb[q] == 0.0;
+  if (all_zero) {  // linalg.py:128-129
+    st.zero_rhs = 1;
+    st.converged = 1;
+    if (stats) *stats = st;
+    if (rel_res) for (int q = 0; q < nrhs; ++q) rel_res[q] = 0;
+    return FDFD_OK;
+  }
+  // transpose: A^T = D A D^-1, solve A x' = D^-1 b, x = D x'
+  double *bb = sw.r;  // current residual, starts as the (scaled) rhs
+  HIP_TRY(hipMemcpyAsync(bb, b, 16 * M * nrhs, hipMemcpyDeviceToDevice, stream));
+  if (transpose) {
+    if (int rc = fdfd_stretch_scale(g, bb, nrhs, 1, stream)) return rc;
+  }
+  // the scaled rhs is needed for every residual: keep it in x's slot? No -
+  // residual r = b' - A x' is formed from b' kept in the caller's b when not
+  // transposed; for transpose a second copy lives after the iterate.
+  std::vector<double> rel(nrhs, 0.0);
+  int stall_count = 0;
+  int rc = FDFD_OK;
+  // b' must persist: use x as the iterate and rebuild r each iteration.
+  // For transpose, b' = D^-1 b is recomputed from b into r before each residual.
+  for (int it = 1; it <= max_iter; ++it) {
+    rc = precision == FDFD_PREC_FP64 ? sweep_impl<double>(g, op, factor, bb, x, nrhs, sw.y, stream)
+                                     : sweep_impl<float>(g, op, factor, bb, x, nrhs, sw.y, stream);
+    if (rc) return rc;
+    st.iterations = it;
+    if (transpose) {
+      HIP_TRY(hipMemcpyAsync(bb, b, 16 * M * nrhs, hipMemcpyDeviceToDevice, stream));
+      if ((rc = fdfd_stretch_scale(g, bb, nrhs, 1, stream))) return rc;
+      hipLaunchKernelGGL(k_stencil, dim3((g->ny + NT - 1) / NT, g->nx, nrhs), dim3(NT), 0, stream,
+                         g->nx, g->ny, op, (const double *)x, (const double *)bb, bb);
+    } else {
+      hipLaunchKernelGGL(k_stencil, dim3((g->ny + NT - 1) / NT, g->nx, nrhs), dim3(NT), 0, stream,
+                         g->nx, g->ny, op, (const double *)x, b, bb);
+    }
+    HIP_TRY(hipGetLastError());
+    if ((rc = norms_async(g, bb, nrhs, transpose, sw.part, sw.norms, stream))) return rc;
+    HIP_TRY(hipMemcpyAsync(nr.data(), sw.norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    double worst = 0;
+    bool contracting = false;
+    for (int q = 0; q < nrhs; ++q) {
+      rel[q] = nb[q] > 0 ? sqrt(nr[q] / nb[q]) : 0.0;
+      if (!std::isfinite(rel[q])) rel[q] = INFINITY;
+      worst = std::max(worst, rel[q]);
+      if (rel[q] > tol && rel[q] < 0.7 * prev[q]) contracting = true;
+    }
+    st.max_rel_residual = worst;
+    if (worst <= tol) { st.converged = 1; break; }
+    if (!std::isfinite(worst)) { st.stalled = 1; break; }
+    stall_count = contracting ? 0 : stall_count + 1;
+    if (stall_count >= 2) { st.stalled = 1; break; }
+    for (int q = 0; q < nrhs; ++q) prev[q] = rel[q];
+  }
+  if (transpose) {
+    if ((rc = fdfd_stretch_scale(g, x, nrhs, 0, stream))) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+  }
+  if (stats) *stats = st;
+  if (rel_res) for (int q = 0; q < nrhs; ++q) rel_res[q] = rel[q];
+  if (!st.converged)
+    return fail(FDFD_ERR_NOT_CONVERGED, "refinement reached %.3e after %d sweeps (tol %.1e)%s",
+                st.max_rel_residual, st.iterations, tol, st.stalled ? ", stalled" : "");
+  return FDFD_OK;
+}
+
+int fdfd_grad_eps(const fdfd_grid *g, const double *lam, const double *e, double *grad,
+                  hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (g->pol != FDFD_POL_EZ) return fail(FDFD_ERR_ARG, "fdfd_grad_eps supports pol='Ez' only");
+  if (!lam || !e || !grad) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_grad_eps");
+  const size_t M = (size_t)g->nx * g->ny;
+  hipLaunchKernelGGL(k_grad_eps, dim3((M + NT - 1) / NT), dim3(NT), 0, stream, *g, lam, e, grad);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+int fdfd_kerr_update(const fdfd_grid *g, double *op, const double *base_c, const double *e,
+                     const double *chi_region, double *eps_nl_out, hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (g->pol != FDFD_POL_EZ) return fail(FDFD_ERR_ARG, "Kerr update supports pol='Ez' only");
+  if (!op || !base_c || !e || !chi_region) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_kerr_update");
+  const size_t M = (size_t)g->nx * g->ny;
+  hipLaunchKernelGGL(k_kerr, dim3((M + NT - 1) / NT), dim3(NT), 0, stream, *g, op, base_c, e,
+                     chi_region, eps_nl_out);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+int fdfd_fields(const fdfd_grid *g, const double *eps, const double *x, double *f1, double *f2,
+                int nrhs, int averaging, hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!x || !f1 || !f2 || nrhs < 1 || (g->pol == FDFD_POL_HZ && !eps))
+    return fail(FDFD_ERR_ARG, "bad arguments to fdfd_fields");
+  hipLaunchKernelGGL(k_fields, dim3((g->ny + NT - 1) / NT, g->nx, nrhs), dim3(NT), 0, stream, *g,
+                     eps, x, f1, f2, averaging);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+int fdfd_selftest_mfma(int precision, const double *a, const double *b, double *d, hipStream_t stream) {
+  if (precision < 0 || precision > 2) return fail(FDFD_ERR_ARG, "selftest mode must be 0, 1 or 2");
+  if (!a || !b || !d) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_selftest_mfma");
+  if (precision == FDFD_PREC_FP64)
+    hipLaunchKernelGGL(k_selftest_mfma_f64, dim3(1), dim3(64), 0, stream, a, b, d, 0);
+  else if (precision == 2)
+    hipLaunchKernelGGL(k_selftest_mfma_f64, dim3(1), dim3(64), 0, stream, a, b, d, 1);
+  else
+    hipLaunchKernelGGL(k_selftest_mfma_f32, dim3(1), dim3(64), 0, stream, a, b, d);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+}  // extern "C"

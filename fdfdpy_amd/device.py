@@ -1,0 +1,243 @@
+"""Device-resident FDFD operator with its factors (host side of the C ABI).
+
+One ``DeviceOperator`` holds, in HBM, what fdfdpy keeps as ``Simulation.A``
+(linalg.py:41-107) -- here the 5-plane band storage written by the assembly
+kernel -- plus the factors that fdfdpy rebuilds and frees on every
+``solver_direct`` call (linalg.py:135-138). Factors stay resident until the
+operator changes, so further sources, adjoint solves and Born re-solves reuse
+them.
+
+Orientation: the block sweep runs along one axis with blocks along the
+other. The operator of the transposed grid is the same physics with x and y
+exchanged (construct_A treats both axes alike), so the grid is transposed on
+the device when that gives a damped sweep axis (NPML >= 1) or a smaller block.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _lib as L
+
+MAX_BLOCK = 768
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def choose_transpose(Nx, Ny, NPML):
+    """True when the sweep should run along y (blocks of size Nx)."""
+    def score(n_block, npml_sweep):
+        if n_block > MAX_BLOCK:
+            return None
+        return (0 if npml_sweep >= 1 else 1, n_block)
+    a = score(Ny, NPML[0])      # sweep along x, blocks of Ny
+    b = score(Nx, NPML[1])      # sweep along y, blocks of Nx
+    if a is None and b is None:
+        raise ValueError(f"grid {Nx}x{Ny}: the shorter side must be <= {MAX_BLOCK} cells")
+    if a is None:
+        return True
+    if b is None:
+        return False
+    return b < a
+
+
+@dataclass
+class SolveReport:
+    precision: str
+    iterations: int
+    converged: bool
+    max_rel_residual: float
+    rel_residual: list
+    fell_back: bool = False
+    factor_seconds: float = 0.0
+    solve_seconds: float = 0.0
+    history: list = field(default_factory=list)
+
+
+class DeviceOperator:
+    """Assembled operator + resident factors for one (grid, omega, eps)."""
+
+    def __init__(self, shape, omega, dl, NPML, pol, L0, device=None):
+        Nx, Ny = int(shape[0]), int(shape[1])
+        if pol not in L.POL:
+            raise ValueError("pol must be one of 'Ez' or 'Hz'")
+        self.shape = (Nx, Ny)
+        self.pol = pol
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        if self.device.type != "cuda":
+            raise ValueError("DeviceOperator needs a GPU device")
+        self.transposed = choose_transpose(Nx, Ny, NPML)
+        nx, ny = (Ny, Nx) if self.transposed else (Nx, Ny)
+        npx, npy = (NPML[1], NPML[0]) if self.transposed else (NPML[0], NPML[1])
+        self.grid = L.FdfdGrid(nx, ny, int(npx), int(npy), L.POL[pol], 0, float(omega), float(dl), float(L0))
+        # grid of the caller's orientation (secondary fields)
+        self.grid_user = L.FdfdGrid(Nx, Ny, int(NPML[0]), int(NPML[1]), L.POL[pol], 0,
+                                    float(omega), float(dl), float(L0))
+        self.lib = L.lib()
+        nbytes = self.lib.fdfd_op_bytes(ctypes.byref(self.grid))
+        if nbytes == 0:
+            raise ValueError(self.lib.fdfd_last_error().decode())
+        self.op = torch.empty(nbytes // 8, dtype=torch.float64, device=self.device)
+        self.eps = None          # complex128, solver orientation
+        self.eps_user = None     # complex128, caller orientation
+        self.base_c = None       # linear diagonal plane (for Kerr updates)
+        self.factors = {}        # precision -> uint8 tensor
+        self.factor_seconds = {}
+
+    # -- sizes ----------------------------------------------------------
+    def factor_bytes(self, precision):
+        return int(self.lib.fdfd_factor_bytes(ctypes.byref(self.grid), L.PREC[precision]))
+
+    def op_bytes(self):
+        return self.op.numel() * 8
+
+    # -- orientation helpers ---------------------------------------------
+    def to_solver(self, v):
+        """(..., Nx, Ny) caller layout -> contiguous solver layout."""
+        return (v.transpose(-1, -2) if self.transposed else v).contiguous()
+
+    def to_user(self, v):
+        return (v.transpose(-1, -2) if self.transposed else v).contiguous()
+
+    # -- assembly ----------------------------------------------------------
+    def set_eps(self, eps):
+        """Assemble A for a new permittivity (the eps_r setter, simulation.py:79-89)."""
+        e = torch.as_tensor(eps, device=self.device)
+        if e.shape != self.shape:
+            raise ValueError(f"eps_r has shape {tuple(e.shape)}, expected {self.shape}")
+        e = e.to(torch.complex128)
+        self.eps_user = e.contiguous()
+        self.eps = self.to_solver(e)
+        L.check(self.lib.fdfd_assemble(ctypes.byref(self.grid), _vp(self.eps), _vp(self.op),
+                                       _stream(self.device)))
+        M = self.shape[0] * self.shape[1]
+        self.base_c = self.op[: 2 * M].clone()
+        self.factors.clear()
+        self.factor_seconds.clear()
+
+    def restore_linear(self):
+        """Undo Kerr diagonal updates (the linear operator is kept, as Simulation.A is)."""
+        M = self.shape[0] * self.shape[1]
+        self.op[: 2 * M].copy_(self.base_c)
+
+    # -- factor --------------------------------------------------------------
+    def factor(self, precision="fp64"):
+        if precision in self.factors:
+            return self.factors[precision]
+        g = ctypes.byref(self.grid)
+        p = L.PREC[precision]
+        fac = torch.empty(self.lib.fdfd_factor_bytes(g, p), dtype=torch.uint8, device=self.device)
+        work = torch.empty(self.lib.fdfd_factor_workspace_bytes(g, p), dtype=torch.uint8, device=self.device)
+        s = _stream(self.device)
+        t0 = time.perf_counter()
+        L.check(self.lib.fdfd_factor(g, _vp(self.op), _vp(fac), p, _vp(work), s))
+        L.check(self.lib.fdfd_factor_status(g, p, _vp(work), s))
+        self.factor_seconds[precision] = time.perf_counter() - t0
+        self.factors[precision] = fac
+        return fac
+
+    def drop_factor(self, precision=None):
+        if precision is None:
+            self.factors.clear()
+        else:
+            self.factors.pop(precision, None)
+
+    # -- solve ---------------------------------------------------------------
+    def solve(self, b, precision="fp64", tol=1e-12, max_iter=60, transpose=False,
+              fallback=True):
+        """x with A x = b (or A^T x = b), b: (nrhs, Nx, Ny) or (Nx, Ny) complex.
+
+        precision 'c64' factors in complex64 and refines in FP64; when the
+        refinement stalls it falls back to the FP64 factor.
+        """
+        squeeze = b.dim() == 2
+        bb = b.reshape((1,) + self.shape) if squeeze else b
+        nrhs = bb.shape[0]
+        bs = self.to_solver(bb.to(device=self.device, dtype=torch.complex128))
+        x = torch.empty_like(bs)
+        rep = self._solve_one(bs, x, nrhs, precision, tol, max_iter, transpose)
+        if not rep.converged and precision == "c64" and fallback:
+            rep64 = self._solve_one(bs, x, nrhs, "fp64", tol, max_iter, transpose)
+            rep64.fell_back = True
+            rep64.factor_seconds += rep.factor_seconds
+            rep64.solve_seconds += rep.solve_seconds
+            rep = rep64
+        if not rep.converged:
+            raise L.NotConverged(L.FDFD_ERR_NOT_CONVERGED,
+                                 f"solve did not reach rel. residual {tol:g}: {rep.max_rel_residual:.3e} "
+                                 f"after {rep.iterations} sweeps ({rep.precision})")
+        out = self.to_user(x)
+        return (out[0] if squeeze else out), rep
+
+    def _solve_one(self, bs, x, nrhs, precision, tol, max_iter, transpose):
+        t0 = time.perf_counter()
+        fac = self.factor(precision)
+        tf = time.perf_counter() - t0 if precision not in self.factor_seconds else self.factor_seconds[precision]
+        g = ctypes.byref(self.grid)
+        p = L.PREC[precision]
+        work = torch.empty(self.lib.fdfd_solve_workspace_bytes(g, p, nrhs), dtype=torch.uint8,
+                           device=self.device)
+        st = L.FdfdSolveStats()
+        rel = (ctypes.c_double * nrhs)()
+        t1 = time.perf_counter()
+        rc = self.lib.fdfd_solve(g, _vp(self.op), _vp(fac), p, _vp(bs), _vp(x), nrhs, int(bool(transpose)),
+                                 float(tol), int(max_iter), _vp(work), ctypes.byref(st), rel,
+                                 _stream(self.device))
+        if rc not in (L.FDFD_OK, L.FDFD_ERR_NOT_CONVERGED):
+            L.check(rc)
+        return SolveReport(precision=precision, iterations=st.iterations, converged=bool(st.converged),
+                           max_rel_residual=st.max_rel_residual, rel_residual=list(rel),
+                           factor_seconds=tf, solve_seconds=time.perf_counter() - t1)
+
+    # -- stencil -------------------------------------------------------------
+    def apply(self, x):
+        """A x for (nrhs, Nx, Ny) or (Nx, Ny) complex x (caller layout)."""
+        squeeze = x.dim() == 2
+        xx = x.reshape((1,) + self.shape) if squeeze else x
+        xs = self.to_solver(xx.to(device=self.device, dtype=torch.complex128))
+        y = torch.empty_like(xs)
+        L.check(self.lib.fdfd_apply(ctypes.byref(self.grid), _vp(self.op), _vp(xs), _vp(y), xs.shape[0],
+                                    _stream(self.device)))
+        out = self.to_user(y)
+        return out[0] if squeeze else out
+
+    def fields(self, x, eps_total=None, averaging=True):
+        """Secondary fields (simulation.py:133-175) of primary fields x (caller layout)."""
+        squeeze = x.dim() == 2
+        xx = (x.reshape((1,) + self.shape) if squeeze else x).to(torch.complex128).contiguous()
+        f1 = torch.empty_like(xx)
+        f2 = torch.empty_like(xx)
+        e = self.eps_user if eps_total is None else eps_total.to(torch.complex128).contiguous()
+        L.check(self.lib.fdfd_fields(ctypes.byref(self.grid_user), _vp(e), _vp(xx), _vp(f1), _vp(f2),
+                                     xx.shape[0], int(bool(averaging)), _stream(self.device)))
+        if squeeze:
+            return f1[0], f2[0]
+        return f1, f2
+
+    def kerr_update(self, e, chi_region):
+        """op.C = linear C + omega^2 eps0 L0 chi_region |e|^2; returns eps_nl (caller layout)."""
+        es = self.to_solver(e.to(torch.complex128))
+        chi = self.to_solver(chi_region.to(device=self.device, dtype=torch.float64))
+        enl = torch.empty(chi.shape, dtype=torch.float64, device=self.device)
+        L.check(self.lib.fdfd_kerr_update(ctypes.byref(self.grid), _vp(self.op), _vp(self.base_c), _vp(es),
+                                          _vp(chi), _vp(enl), _stream(self.device)))
+        return self.to_user(enl)
+
+    def grad_eps(self, lam, e):
+        """-Re(lam * omega^2 eps0 L0 * e) (Ez), both in caller layout."""
+        lam = lam.to(torch.complex128).contiguous()
+        e = e.to(torch.complex128).contiguous()
+        out = torch.empty(self.shape, dtype=torch.float64, device=self.device)
+        L.check(self.lib.fdfd_grad_eps(ctypes.byref(self.grid_user), _vp(lam), _vp(e), _vp(out),
+                                       _stream(self.device)))
+        return out

@@ -1,0 +1,165 @@
+/*
+ * fdfd_hip.h -- C ABI of the MI355X (gfx950) FDFD solve path.
+ *
+ * The library replaces the numeric core of fdfdpy's 2D solve path:
+ *   operator assembly  fdfdpy/linalg.py:41-107  (construct_A, with
+ *                      fdfdpy/pml.py:7-89 S_create/create_sfactor and
+ *                      fdfdpy/derivatives.py:7-34 createDws)
+ *   direct solve       fdfdpy/linalg.py:122-149 (solver_direct: pyMKL
+ *                      pardisoSolver(A, mtype=13).factor()/.solve(b)/.clear()
+ *                      or scipy spsolve)
+ *   mat-vecs           the scipy CSR products A.dot(x) used by
+ *                      fdfdpy/nonlinear_solvers.py:131-133
+ *   Kerr diagonal      fdfdpy/simulation.py:57-68 (compute_nl -> Anl)
+ *
+ * Conventions
+ *   - Every function returns an int status: FDFD_OK (0) or an FDFD_ERR_* code;
+ *     fdfd_last_error() returns the text of the calling thread's last error.
+ *     No C++ exception crosses this boundary.
+ *   - All array arguments are device pointers owned by the caller; the library
+ *     allocates no device memory. Complex arrays are interleaved (re, im)
+ *     doubles, C order, x the slow (block) index: v[r][i][j] with
+ *     0 <= r < nrhs, 0 <= i < nx, 0 <= j < ny.
+ *   - Work is enqueued on the given stream; only fdfd_solve and fdfd_norms
+ *     wait for the stream (they read convergence data on the host).
+ *   - The operator is stored as 5 complex planes [5][nx][ny]: the diagonal and
+ *     the couplings to (i-1,j), (i+1,j), (i,j-1), (i,j+1), indices periodic.
+ */
+#ifndef FDFD_HIP_H
+#define FDFD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDFD_ABI_VERSION 1
+
+enum {
+  FDFD_OK = 0,
+  FDFD_ERR_ARG = 1,           /* invalid argument (mirrors ValueError)      */
+  FDFD_ERR_HIP = 2,           /* a HIP runtime call failed                  */
+  FDFD_ERR_NOT_CONVERGED = 3, /* refinement did not reach the tolerance     */
+  FDFD_ERR_NUMERIC = 4        /* non-finite or zero pivot in the factor     */
+};
+
+enum { FDFD_POL_EZ = 0, FDFD_POL_HZ = 1 };          /* simulation.py:256 */
+enum { FDFD_PREC_FP64 = 0, FDFD_PREC_C64 = 1 };      /* factor precision  */
+enum { FDFD_OP_C = 0, FDFD_OP_XM = 1, FDFD_OP_XP = 2, FDFD_OP_YM = 3, FDFD_OP_YP = 4 };
+
+/* Grid and physics parameters; the fields of Simulation.__init__
+ * (simulation.py:17-34): omega, dl, NPML, pol, L0 and eps_r.shape. */
+typedef struct fdfd_grid {
+  int32_t nx, ny;         /* cells along x (block/sweep axis) and y         */
+  int32_t npml_x, npml_y; /* NPML[0], NPML[1]                               */
+  int32_t pol;            /* FDFD_POL_EZ or FDFD_POL_HZ                     */
+  int32_t reserved;
+  double omega;           /* angular frequency (rad/s)                      */
+  double dl;              /* cell size in units of L0                       */
+  double L0;              /* length scale (m)                               */
+} fdfd_grid;
+
+/* Refinement report of fdfd_solve (no reference counterpart: Pardiso's
+ * solve returns no residual, linalg.py:137). */
+typedef struct fdfd_solve_stats {
+  int32_t iterations;     /* sweeps applied                                 */
+  int32_t converged;      /* 1 when every RHS met the tolerance             */
+  int32_t stalled;        /* 1 when refinement stopped contracting          */
+  int32_t zero_rhs;       /* 1 when every RHS was zero (shortcut taken)     */
+  double max_rel_residual;/* max over RHS of ||b - A x|| / ||b||            */
+} fdfd_solve_stats;
+
+int fdfd_abi_version(void);
+const char *fdfd_last_error(void);
+
+/* Byte sizes for the caller's allocations (admission uses these). */
+size_t fdfd_op_bytes(const fdfd_grid *g);
+size_t fdfd_factor_bytes(const fdfd_grid *g, int precision);
+size_t fdfd_factor_workspace_bytes(const fdfd_grid *g, int precision);
+size_t fdfd_solve_workspace_bytes(const fdfd_grid *g, int precision, int nrhs);
+int fdfd_block_padding(void);   /* block size is padded to this multiple */
+
+/* Replaces construct_A (linalg.py:41-107) incl. S_create (pml.py:42-89) and
+ * createDws (derivatives.py:7-34): writes the SC-PML 5-point operator for
+ * eps (complex, [nx][ny]) into op (fdfd_op_bytes). Hz uses the grid-averaged
+ * permittivity (linalg.py:71-72, averaging=True as the eps_r setter passes). */
+int fdfd_assemble(const fdfd_grid *g, const double *eps, double *op, hipStream_t stream);
+
+/* Replaces A.dot(x) (nonlinear_solvers.py:132): y = A x for nrhs vectors. */
+int fdfd_apply(const fdfd_grid *g, const double *op, const double *x, double *y,
+               int nrhs, hipStream_t stream);
+
+/* r = b - A x for nrhs vectors (the residual of linalg.py:122-149's system). */
+int fdfd_residual(const fdfd_grid *g, const double *op, const double *b,
+                  const double *x, double *r, int nrhs, hipStream_t stream);
+
+/* Replaces pardisoSolver(A, mtype=13).factor() (linalg.py:135-136): block
+ * sweep factorization of the x-non-periodic part of A. factor holds
+ * fdfd_factor_bytes, work fdfd_factor_workspace_bytes. Enqueued on stream;
+ * status is checked by fdfd_factor_status after the stream completes. */
+int fdfd_factor(const fdfd_grid *g, const double *op, void *factor, int precision,
+                void *work, hipStream_t stream);
+
+/* Reads the numeric status the factor kernels left in work (waits for the
+ * stream): FDFD_OK or FDFD_ERR_NUMERIC. */
+int fdfd_factor_status(const fdfd_grid *g, int precision, void *work, hipStream_t stream);
+
+/* One application of the factor: x += F^-1 r (nrhs vectors). */
+int fdfd_sweep(const fdfd_grid *g, const double *op, const void *factor, int precision,
+               const double *r, double *x, int nrhs, void *work, hipStream_t stream);
+
+/* Replaces pardisoSolver.solve(b) (linalg.py:137) and spsolve (linalg.py:141):
+ * solves A x = b (transpose = 0) or A^T x = b (transpose = 1, the adjoint
+ * system) to ||b - A x|| <= tol ||b|| per RHS by refinement with the factor,
+ * at most max_iter sweeps. An all-zero b gives x = 0 without a sweep
+ * (linalg.py:128-129). rel_res (optional, device-independent host array of
+ * nrhs doubles) receives the final relative residuals. */
+int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int precision,
+               const double *b, double *x, int nrhs, int transpose, double tol,
+               int max_iter, void *work, fdfd_solve_stats *stats, double *rel_res,
+               hipStream_t stream);
+
+/* Per-RHS squared 2-norms of v (device) into host out[nrhs]; waits. */
+int fdfd_norms(const fdfd_grid *g, const double *v, int nrhs, void *work,
+               double *out, hipStream_t stream);
+
+/* v *= diag(sxf_i * syf_j) (inverse = 0) or its inverse (inverse = 1): the
+ * symmetrizer with D A = (D A)^T, so A^T = D A D^-1 (pml.py:42-89). */
+int fdfd_stretch_scale(const fdfd_grid *g, double *v, int nrhs, int inverse,
+                       hipStream_t stream);
+
+/* Adjoint gradient of a real objective w.r.t. eps (Ez): grad[i][j] =
+ * -Re(lam * omega^2 eps0 L0 * e), lam = A^-T dJ/dE (no reference
+ * counterpart; dA/deps follows linalg.py:53-66). grad is real [nx][ny]. */
+int fdfd_grad_eps(const fdfd_grid *g, const double *lam, const double *e,
+                  double *grad, hipStream_t stream);
+
+/* Kerr update of the diagonal (simulation.py:57-68, nonlinearity.py:28-34):
+ * eps_nl = chi_region * |e|^2 (chi_region = 3 chi nl_region [* scale], real),
+ * op.C = base_c + omega^2 eps0 L0 eps_nl. eps_nl_out (real [nx][ny]) may be
+ * NULL. */
+int fdfd_kerr_update(const fdfd_grid *g, double *op, const double *base_c,
+                     const double *e, const double *chi_region, double *eps_nl_out,
+                     hipStream_t stream);
+
+/* Secondary fields of solve_fields (simulation.py:133-175) for nrhs primary
+ * fields x: Ez -> f1 = Hx, f2 = Hy; Hz -> f1 = Ex, f2 = Ey, where Hz uses eps
+ * (the total permittivity, eps_r + eps_nl) with grid averaging when
+ * averaging != 0 (simulation.py:134-141). eps may be NULL for Ez. */
+int fdfd_fields(const fdfd_grid *g, const double *eps, const double *x, double *f1, double *f2,
+                int nrhs, int averaging, hipStream_t stream);
+
+/* Check of the MFMA operand layouts the factor kernels rely on (no reference
+ * counterpart): D = A (16x4) * B (4x16) with the 16x16x4 f64 (precision 0)
+ * or f32 (precision 1) instruction; a, b, d are row-major device arrays of
+ * doubles (converted to float for precision 1). */
+int fdfd_selftest_mfma(int precision, const double *a, const double *b, double *d,
+                       hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDFD_HIP_H */
