@@ -27,7 +27,20 @@ MAX_BLOCK = 768
 
 
 def _vp(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+    """Raw device pointer of a tensor handed to the C ABI.
+
+    The ABI reads memory as it lies, so lazily conjugated/negated views and
+    strided views are refused here rather than read wrongly.
+    """
+    if t is None:
+        return ctypes.c_void_p(0)
+    if t.is_conj() or t.is_neg() or not t.is_contiguous():
+        raise ValueError("tensor passed to the C ABI must be contiguous with no conj/neg view bits")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _dense(t):
+    return t.resolve_conj().resolve_neg().contiguous()
 
 
 def _stream(device):
@@ -91,7 +104,8 @@ class DeviceOperator:
         self.eps = None          # complex128, solver orientation
         self.eps_user = None     # complex128, caller orientation
         self.base_c = None       # linear diagonal plane (for Kerr updates)
-        self.factors = {}        # precision -> uint8 tensor
+        self.version = 0         # bumped whenever the operator's diagonal changes
+        self.factors = {}        # precision -> (uint8 tensor, operator version)
         self.factor_seconds = {}
 
     # -- sizes ----------------------------------------------------------
@@ -104,10 +118,10 @@ class DeviceOperator:
     # -- orientation helpers ---------------------------------------------
     def to_solver(self, v):
         """(..., Nx, Ny) caller layout -> contiguous solver layout."""
-        return (v.transpose(-1, -2) if self.transposed else v).contiguous()
+        return _dense(v.transpose(-1, -2) if self.transposed else v)
 
     def to_user(self, v):
-        return (v.transpose(-1, -2) if self.transposed else v).contiguous()
+        return _dense(v.transpose(-1, -2) if self.transposed else v)
 
     # -- assembly ----------------------------------------------------------
     def set_eps(self, eps):
@@ -116,24 +130,32 @@ class DeviceOperator:
         if e.shape != self.shape:
             raise ValueError(f"eps_r has shape {tuple(e.shape)}, expected {self.shape}")
         e = e.to(torch.complex128)
-        self.eps_user = e.contiguous()
+        self.eps_user = _dense(e)
         self.eps = self.to_solver(e)
         L.check(self.lib.fdfd_assemble(ctypes.byref(self.grid), _vp(self.eps), _vp(self.op),
                                        _stream(self.device)))
         M = self.shape[0] * self.shape[1]
         self.base_c = self.op[: 2 * M].clone()
+        self.version += 1
+        self.linear_version = self.version
         self.factors.clear()
         self.factor_seconds.clear()
 
     def restore_linear(self):
         """Undo Kerr diagonal updates (the linear operator is kept, as Simulation.A is)."""
-        M = self.shape[0] * self.shape[1]
-        self.op[: 2 * M].copy_(self.base_c)
+        if self.version != self.linear_version:
+            M = self.shape[0] * self.shape[1]
+            self.op[: 2 * M].copy_(self.base_c)
+            self.version = self.linear_version
 
     # -- factor --------------------------------------------------------------
-    def factor(self, precision="fp64"):
+    def factor(self, precision="fp64", refresh=False):
+        """Resident factor for this precision; built for the current operator
+        when absent or when refresh is set and it belongs to an older one."""
         if precision in self.factors:
-            return self.factors[precision]
+            fac, ver = self.factors[precision]
+            if not refresh or ver == self.version:
+                return fac
         g = ctypes.byref(self.grid)
         p = L.PREC[precision]
         fac = torch.empty(self.lib.fdfd_factor_bytes(g, p), dtype=torch.uint8, device=self.device)
@@ -143,7 +165,7 @@ class DeviceOperator:
         L.check(self.lib.fdfd_factor(g, _vp(self.op), _vp(fac), p, _vp(work), s))
         L.check(self.lib.fdfd_factor_status(g, p, _vp(work), s))
         self.factor_seconds[precision] = time.perf_counter() - t0
-        self.factors[precision] = fac
+        self.factors[precision] = (fac, self.version)
         return fac
 
     def drop_factor(self, precision=None):
@@ -166,6 +188,14 @@ class DeviceOperator:
         bs = self.to_solver(bb.to(device=self.device, dtype=torch.complex128))
         x = torch.empty_like(bs)
         rep = self._solve_one(bs, x, nrhs, precision, tol, max_iter, transpose)
+        if not rep.converged and self.factors[precision][1] != self.version:
+            # the resident factor belongs to an earlier diagonal (Born steps):
+            # refactor the current operator and try again
+            self.factor(precision, refresh=True)
+            rep2 = self._solve_one(bs, x, nrhs, precision, tol, max_iter, transpose)
+            rep2.factor_seconds += rep.factor_seconds
+            rep2.solve_seconds += rep.solve_seconds
+            rep = rep2
         if not rep.converged and precision == "c64" and fallback:
             rep64 = self._solve_one(bs, x, nrhs, "fp64", tol, max_iter, transpose)
             rep64.fell_back = True
@@ -214,10 +244,10 @@ class DeviceOperator:
     def fields(self, x, eps_total=None, averaging=True):
         """Secondary fields (simulation.py:133-175) of primary fields x (caller layout)."""
         squeeze = x.dim() == 2
-        xx = (x.reshape((1,) + self.shape) if squeeze else x).to(torch.complex128).contiguous()
+        xx = _dense((x.reshape((1,) + self.shape) if squeeze else x).to(torch.complex128))
         f1 = torch.empty_like(xx)
         f2 = torch.empty_like(xx)
-        e = self.eps_user if eps_total is None else eps_total.to(torch.complex128).contiguous()
+        e = self.eps_user if eps_total is None else _dense(eps_total.to(torch.complex128))
         L.check(self.lib.fdfd_fields(ctypes.byref(self.grid_user), _vp(e), _vp(xx), _vp(f1), _vp(f2),
                                      xx.shape[0], int(bool(averaging)), _stream(self.device)))
         if squeeze:
@@ -226,6 +256,7 @@ class DeviceOperator:
 
     def kerr_update(self, e, chi_region):
         """op.C = linear C + omega^2 eps0 L0 chi_region |e|^2; returns eps_nl (caller layout)."""
+        self.version += 1
         es = self.to_solver(e.to(torch.complex128))
         chi = self.to_solver(chi_region.to(device=self.device, dtype=torch.float64))
         enl = torch.empty(chi.shape, dtype=torch.float64, device=self.device)
@@ -235,8 +266,8 @@ class DeviceOperator:
 
     def grad_eps(self, lam, e):
         """-Re(lam * omega^2 eps0 L0 * e) (Ez), both in caller layout."""
-        lam = lam.to(torch.complex128).contiguous()
-        e = e.to(torch.complex128).contiguous()
+        lam = _dense(lam.to(torch.complex128))
+        e = _dense(e.to(torch.complex128))
         out = torch.empty(self.shape, dtype=torch.float64, device=self.device)
         L.check(self.lib.fdfd_grad_eps(ctypes.byref(self.grid_user), _vp(lam), _vp(e), _vp(out),
                                        _stream(self.device)))

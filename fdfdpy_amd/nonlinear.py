@@ -1,0 +1,108 @@
+"""Kerr nonlinearity and the Born iteration on the device.
+
+Mirrors fdfdpy/nonlinearity.py:6-37 (the Kerr model), Simulation.compute_nl
+(simulation.py:57-68) and born_solve (nonlinear_solvers.py:13-54). Each Born
+step rewrites the operator's diagonal on the device (fdfd_kerr_update) and
+re-solves; the resident factor of the linear operator serves as the
+refinement preconditioner, and the operator is refactored only when that
+refinement stops contracting.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+class Nonlinearity:
+    """Kerr nonlinearity (nonlinearity.py:6-37).
+
+    ``chi`` is the internal coefficient (Simulation.add_nl divides the SI
+    chi3 by L0**2, simulation.py:72). eps_nl = 3 chi nl_region |e|^2, scaled
+    by (eps_r - 1)/(eps_max - 1) when eps_scale is set.
+    """
+
+    def __init__(self, chi, nl_region, nl_type='kerr', eps_scale=False, eps_max=None):
+        self.chi = chi
+        self.nl_region = nl_region
+        self.nl_type = nl_type
+        self.eps_scale = eps_scale
+        self.eps_max = eps_max
+        if nl_type != 'kerr':
+            raise AssertionError("Only 'kerr' type nonlinearity is supported")
+        if eps_scale and eps_max is None:
+            raise AssertionError("Must provide eps_max when eps_scale is True")
+
+    def coefficient(self, eps_r):
+        """Real array c with eps_nl = c |e|^2."""
+        c = 3 * self.chi * np.asarray(self.nl_region, dtype=np.float64)
+        if self.eps_scale:
+            c = c * ((np.real(np.asarray(eps_r)) - 1) / (self.eps_max - 1))
+        return c
+
+    def eps_nl(self, e, eps_r):
+        return self.coefficient(eps_r) * np.square(np.abs(e))
+
+    def dnl_de(self, e, eps_r):
+        c = 3 * self.chi * np.asarray(self.nl_region)
+        if self.eps_scale:
+            c = c * ((np.real(np.asarray(eps_r)) - 1) / (self.eps_max - 1))
+        return c * np.conj(e)
+
+
+def kerr_coefficient(sim):
+    """Sum of the Kerr coefficients of all nonlinearities of sim (device tensor)."""
+    eps = sim.eps_r.detach().cpu().numpy() if isinstance(sim.eps_r, torch.Tensor) else sim.eps_r
+    c = np.zeros((sim.Nx, sim.Ny))
+    for nli in sim.nonlinearity:
+        c = c + nli.coefficient(eps)
+    return torch.as_tensor(c, dtype=torch.float64, device=sim.device)
+
+
+def compute_nl(sim, e):
+    """eps_nl of field e and the nonlinear diagonal on the device (simulation.py:57-68)."""
+    et = torch.as_tensor(e, device=sim.device).to(torch.complex128)
+    enl = sim.operator.kerr_update(et, kerr_coefficient(sim))
+    sim.eps_nl = enl.cpu().numpy()
+    return enl
+
+
+def born_solve(sim, Estart=None, conv_threshold=1e-10, max_num_iter=50, averaging=True,
+               precision='fp64', tol=1e-12):
+    """Born fixed point E_{k+1} = (A + Anl(E_k))^-1 b (nonlinear_solvers.py:13-54)."""
+    conv_array = np.zeros((max_num_iter, 1))
+    op = sim.operator
+    if Estart is None:
+        if sim.fields['Ez'] is None:
+            (_, _, Ez) = sim.solve_fields(precision='fp64' if precision == 'fp64' else 'mixed')
+        else:
+            Ez = sim.fields['Ez']
+    else:
+        Ez = Estart
+    E = torch.as_tensor(Ez, device=sim.device).to(torch.complex128).clone()
+    coeff = kerr_coefficient(sim)
+    b = sim._src_tensor() * (1j * sim.omega)
+    convergence = np.inf
+    enl = None
+    try:
+        for istep in range(max_num_iter):
+            Eprev = E
+            enl = op.kerr_update(Eprev, coeff)
+            if bool(torch.any(b != 0)):
+                E, rep = op.solve(b, precision=precision, tol=tol)
+                sim.last_report = rep
+            else:
+                E = torch.zeros_like(b)
+            nE = torch.linalg.vector_norm(E).item()
+            convergence = torch.linalg.vector_norm(E - Eprev).item() / nE if nE > 0 else 0.0
+            conv_array[istep] = convergence
+            if convergence < conv_threshold:
+                break
+    finally:
+        if enl is not None:
+            sim.eps_nl = enl.cpu().numpy()
+        op.restore_linear()
+    if convergence > conv_threshold:
+        print("the simulation did not converge, reached {}".format(convergence))
+    Hx, Hy = op.fields(E)
+    return sim._out(Hx), sim._out(Hy), sim._out(E), conv_array

@@ -1,0 +1,186 @@
+"""GPU parity of the HIP path against the oracle and the golden fixtures."""
+
+import ctypes
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import GOLDEN
+from fdfdpy_amd import _lib as L
+from fdfdpy_amd import configs as C
+from fdfdpy_amd.device import DeviceOperator
+from fdfdpy_amd.simulation import Simulation
+from oracle import OracleSimulation, adjoint_gradient, solver_direct
+
+pytestmark = pytest.mark.gpu
+OMEGA = C.omega_of(1.55e-6)
+DEV = "cuda:0"
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mfma_lane_layout(mode):
+    rng = np.random.default_rng(mode)
+    a = torch.tensor(rng.standard_normal((16, 4)), device=DEV)
+    b = torch.tensor(rng.standard_normal((4, 16)), device=DEV)
+    d = torch.zeros((16, 16), dtype=torch.float64, device=DEV)
+    lib = L.lib()
+    L.check(lib.fdfd_selftest_mfma(mode, ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                                   ctypes.c_void_p(d.data_ptr()), None))
+    torch.cuda.synchronize()
+    tol = 1e-13 if mode == 0 else 1e-5
+    assert np.abs(d.cpu().numpy() - (a @ b).cpu().numpy()).max() < tol
+
+
+@pytest.mark.parametrize("pol,N,npml", [
+    ("Ez", (40, 37), [8, 6]), ("Hz", (36, 45), [7, 9]), ("Ez", (30, 50), [0, 6]),
+    ("Hz", (64, 64), [10, 10]), ("Ez", (33, 17), [2, 3]),
+])
+def test_assembly_matches_oracle_operator(pol, N, npml):
+    rng = np.random.default_rng(1)
+    eps = 1 + 11.25 * (rng.random(N) < 0.5)
+    sim = OracleSimulation(OMEGA, eps, 0.02, npml, pol)
+    op = DeviceOperator(N, OMEGA, 0.02, npml, pol, 1e-6, DEV)
+    op.set_eps(eps)
+    x = rng.standard_normal((3,) + N) + 1j * rng.standard_normal((3,) + N)
+    y = op.apply(torch.tensor(x, device=DEV)).cpu().numpy()
+    for q in range(3):
+        yref = (sim.A @ x[q].reshape(-1)).reshape(N)
+        assert relerr(y[q], yref) < 1e-14
+
+
+@pytest.mark.parametrize("pol,N,npml", [
+    ("Ez", (40, 37), [8, 6]), ("Hz", (36, 45), [7, 9]), ("Ez", (30, 50), [0, 6]), ("Ez", (70, 64), [10, 10]),
+])
+@pytest.mark.parametrize("precision", ["fp64", "c64"])
+def test_solve_matches_oracle(pol, N, npml, precision):
+    rng = np.random.default_rng(2)
+    eps = 1 + 11.25 * (rng.random(N) < 0.5)
+    sim = OracleSimulation(OMEGA, eps, 0.02, npml, pol)
+    op = DeviceOperator(N, OMEGA, 0.02, npml, pol, 1e-6, DEV)
+    op.set_eps(eps)
+    src = np.zeros((2,) + N)
+    src[0, N[0] // 2, N[1] // 3] = 1
+    src[1] = rng.standard_normal(N)
+    b = src * 1j * OMEGA
+    x, rep = op.solve(torch.tensor(b, device=DEV), precision=precision, tol=1e-13)
+    assert rep.converged and rep.max_rel_residual <= 1e-13
+    for q in range(2):
+        xref = solver_direct(sim.A, b[q]).reshape(N)
+        assert relerr(x[q].cpu().numpy(), xref) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c1_simulation_matches_golden_and_oracle(precision):
+    p = C.c1_point()
+    g = np.load(os.path.join(GOLDEN, 'c1_field.npz'))
+    sim = Simulation(p.omega, p.eps, p.dl, p.npml, 'Ez')
+    sim.src[60, 60] = 1
+    Hx, Hy, Ez = sim.solve_fields(precision=precision)
+    assert Ez.dtype == np.complex128 and Ez.shape == (120, 120)
+    assert relerr(Ez, g['Ez']) <= 1e-10
+    ref = OracleSimulation(p.omega, p.eps, p.dl, p.npml, 'Ez')
+    ref.src[60, 60] = 1
+    rHx, rHy, rEz = ref.solve_fields()
+    assert relerr(Hx, rHx) <= 1e-10 and relerr(Hy, rHy) <= 1e-10
+    assert sim.fields['Ez'] is Ez
+
+
+def test_hz_simulation_fields_match_oracle():
+    rng = np.random.default_rng(3)
+    N = (48, 56)
+    eps = 1 + 11.25 * (rng.random(N) < 0.5)
+    sim = Simulation(OMEGA, eps, 0.02, [8, 8], 'Hz')
+    ref = OracleSimulation(OMEGA, eps, 0.02, [8, 8], 'Hz')
+    for s in (sim, ref):
+        s.src[20, 30] = 1
+        s.src[10, 5] = -2
+    Ex, Ey, Hz = sim.solve_fields()
+    rEx, rEy, rHz = ref.solve_fields()
+    for a, b in ((Ex, rEx), (Ey, rEy), (Hz, rHz)):
+        assert relerr(a, b) <= 1e-10
+
+
+def test_zero_source_shortcut_and_resident_factor():
+    sim = Simulation(OMEGA, np.ones((40, 40)), 0.02, [8, 8], 'Ez')
+    Hx, Hy, Ez = sim.solve_fields()
+    assert not Ez.any() and Ez.dtype == np.complex128
+    assert sim.operator.factors == {}          # no factor for a zero RHS
+    sim.src[20, 20] = 1
+    sim.solve_fields()
+    fac = sim.operator.factors['fp64'][0]
+    sim.src[10, 10] = 1
+    sim.solve_fields()
+    assert sim.operator.factors['fp64'][0] is fac   # reused
+    sim.eps_r = np.full((40, 40), 2.0)
+    assert sim.operator.factors == {}               # invalidated by the setter
+    with pytest.raises(ValueError):
+        sim.solve_fields(solver='umfpack')
+
+
+@pytest.mark.parametrize("pol", ["Ez", "Hz"])
+def test_transpose_solve_matches_csr_transpose(pol):
+    rng = np.random.default_rng(4)
+    N = (44, 38)
+    eps = 1 + 11.25 * (rng.random(N) < 0.5)
+    sim = OracleSimulation(OMEGA, eps, 0.02, [8, 7], pol)
+    op = DeviceOperator(N, OMEGA, 0.02, [8, 7], pol, 1e-6, DEV)
+    op.set_eps(eps)
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    x, rep = op.solve(torch.tensor(b, device=DEV), tol=1e-13, transpose=True)
+    xref = solver_direct(sim.A.T.tocsr(), b.reshape(-1)).reshape(N)
+    assert relerr(x.cpu().numpy(), xref) < 1e-10
+
+
+def test_born_matches_golden_trajectory():
+    g = np.load(os.path.join(GOLDEN, 'born_small.npz'))
+    N = (64, 64)
+    eps = np.ones(N)
+    eps[24:40, 24:40] = 12.25
+    region = np.zeros(N)
+    region[24:40, 24:40] = 1
+    sim = Simulation(OMEGA, eps, 0.02, [10, 10], 'Ez')
+    sim.src[12, 24:40] = 40.0
+    sim.add_nl(2.8e-18, region)
+    Hx, Hy, Ez, conv = sim.solve_fields_nl(solver_nl='born', conv_threshold=1e-10)
+    assert relerr(Ez, g['Ez']) <= 1e-10
+    nsteps = int(np.count_nonzero(g['conv']))
+    assert np.count_nonzero(conv) == nsteps
+    # the per-step changes follow the reference trajectory until round-off
+    c = conv.ravel()[:nsteps - 1]
+    np.testing.assert_allclose(c, g['conv'][:nsteps - 1], rtol=1e-4)
+    np.testing.assert_allclose(sim.eps_nl, g['eps_nl'], rtol=1e-9, atol=1e-14)
+    # the linear operator is restored afterwards
+    _, _, Elin = sim.solve_fields()
+    ref = OracleSimulation(OMEGA, eps, 0.02, [10, 10], 'Ez')
+    ref.src[12, 24:40] = 40.0
+    _, _, rElin = ref.solve_fields()
+    assert relerr(Elin, rElin) <= 1e-10
+
+
+def test_adjoint_gradient_matches_oracle():
+    from fdfdpy_amd.adjoint import FieldSolver
+    rng = np.random.default_rng(2)
+    N = (40, 36)
+    eps = np.ones(N)
+    eps[14:26, 12:24] = rng.uniform(1, 12.25, size=(12, 12))
+    probe = np.zeros(N)
+    probe[32, 14:22] = 1
+    src = np.zeros(N)
+    src[6, 12:24] = 1
+    solver = FieldSolver(N, OMEGA, 0.02, [8, 8], device=DEV)
+    e = torch.tensor(eps, device=DEV, requires_grad=True)
+    Ez = solver(e, torch.tensor(src, device=DEV))
+    J = (Ez.abs() ** 2 * torch.tensor(probe, device=DEV)).sum()
+    J.backward()
+    ref = OracleSimulation(OMEGA, eps, 0.02, [8, 8], 'Ez')
+    ref.src = src.copy()
+    Jr, gr = adjoint_gradient(ref, probe)
+    assert abs(J.item() - Jr) <= 1e-10 * Jr
+    g = e.grad.cpu().numpy()
+    assert relerr(g, gr) <= 1e-8
