@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""FDFD serving benchmark on one MI355X.
+
+Workload (the serving stream): Ez requests on square grids drawn from
+{128, 192, 256, 384, 512, 768} with NPML 15-25 scaled to the grid, lambda
+~ U[1.3, 1.7] um, dl = 0.02, a seeded Bernoulli design region and 1-4 point
+sources each, arriving as a Poisson process. A step is one batch of
+--batch requests of that stream; --steps batches make the timed stream
+(8 x 64 = the 512-request stream by default).
+
+Metric: solves/s (right-hand sides solved to relative residual <= 1e-10,
+each the equivalent of one reference solve_fields call) with p50/p99
+latency, for the complex64-factor + FP64-refinement path (the headline
+value) and for the FP64-factor path on the same trace.
+
+Also reported: the roofline of the dominant kernel (the Gauss-Jordan step
+of the complex64 factor, timed with HIP events on its stream) and a CPU
+baseline: the oracle's SuperLU restatement of the reference's 'scipy'
+branch on a bounded sample of the same stream.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_MFMA_PEAK_TFLOPS = 157.3
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8, help="timed batches of the stream")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed batches (of 8 requests each)")
+    ap.add_argument("--batch", type=int, default=64, help="requests per step")
+    ap.add_argument("--rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--slots", type=int, default=4, help="requests in flight")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--roofline-only", action="store_true", help="only the dominant-kernel measurement")
+    return ap.parse_args()
+
+
+def roofline_leg(n=512, precision="c64"):
+    """Factor one n x n operator; the factor is k_gj_step launches back to back."""
+    import torch
+    from fdfdpy_amd import configs as C
+    from fdfdpy_amd.device import DeviceOperator
+    p = C.c3_request(7, n=n)
+    op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, "cuda:0")
+    op.set_eps(p.eps)
+    op.factor(precision)            # warm
+    op.drop_factor()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(s)
+    op.factor(precision)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    nt = (n + 31) // 32
+    launches = n * nt + 1
+    flops = 8.0 * n ** 4                       # 8 m n^3 real flops, m = n blocks
+    achieved = flops / (ms * 1e-3) / 1e12
+    peak = FP32_MFMA_PEAK_TFLOPS if precision == "c64" else FP64_MFMA_PEAK_TFLOPS
+    return {
+        "kernel": f"k_gj_step<{'float' if precision == 'c64' else 'double'}>",
+        "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+        "frac": round(achieved / peak, 5), "traffic": None,
+        "workload": f"complex factor of a {n}x{n} Ez operator: {launches} launches, "
+                    f"8*n^4 = {flops:.3e} flops",
+        "avg_launch_us": round(ms * 1e3 / launches, 3), "factor_ms": round(ms, 3),
+    }
+
+
+def cpu_baseline(reqs, seconds):
+    """Oracle (CSR + SuperLU, refactored per RHS as linalg.py:122-149 does) on a prefix of the stream."""
+    from threadpoolctl import threadpool_limits
+    from oracle import construct_A, solver_direct
+    done, rhs, sizes, t0 = 0, 0, [], time.perf_counter()
+    with threadpool_limits(limits=1):
+        for r in reqs:
+            p = r.problem
+            N = p.shape
+            for q in range(p.srcs.shape[0]):
+                A, _ = construct_A(p.omega, [0, N[0] * p.dl], [0, N[1] * p.dl], p.eps, p.npml, p.pol, p.L0)
+                solver_direct(A, p.srcs[q] * 1j * p.omega)
+                rhs += 1
+            done += 1
+            sizes.append(N[0])
+            if time.perf_counter() - t0 >= seconds:
+                break
+    wall = time.perf_counter() - t0
+    return {"value": round(rhs / wall, 4), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} requests of the stream (grids {sizes}), {rhs} solves, "
+                      f"assembly + SuperLU spsolve per solve, {wall:.1f} s on 1 host core"}
+
+
+def run_stream(reqs, precision, slots, tol):
+    import torch
+    from fdfdpy_amd.serving import ServingEngine, summarize
+    eng = ServingEngine("cuda:0", precision=precision, slots=slots, tol=tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = eng.run(reqs)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return summarize(res, wall), res
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design, not built)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from fdfdpy_amd import configs as C
+    from fdfdpy_amd.serving import Request
+
+    if args.roofline_only:
+        print(json.dumps(roofline_leg()))
+        return
+
+    n_req = args.steps * args.batch
+    log(f"stream: {n_req} requests, Poisson rate {args.rate}/s, {args.slots} slots")
+    problems = [C.c3_request(s) for s in range(n_req)]
+    rng = np.random.default_rng(3)
+    gaps = rng.exponential(1.0 / args.rate, size=n_req)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs = [Request(i, problems[i], float(arrivals[i])) for i in range(n_req)]
+
+    # warmup: separate requests of the same stream (8 per warmup step)
+    for w in range(args.warmup):
+        wreqs = [Request(i, C.c3_request(100000 + 8 * w + i), 0.0) for i in range(8)]
+        for prec in (("c64",) if args.no_fp64 else ("c64", "fp64")):
+            s, _ = run_stream(wreqs, prec, args.slots, args.tol)
+            log(f"warmup {w} {prec}: {s['solves']} solves in {s['wall_s']:.2f}s")
+
+    mixed, res_m = run_stream(reqs, "c64", args.slots, args.tol)
+    log(f"mixed: {json.dumps(mixed)}")
+    fp64 = None
+    if not args.no_fp64:
+        fp64, res_f = run_stream(reqs, "fp64", args.slots, args.tol)
+        log(f"fp64: {json.dumps(fp64)}")
+    roof = roofline_leg()
+    log(f"roofline: {json.dumps(roof)}")
+    cpu = None if args.no_cpu_baseline else cpu_baseline(reqs, args.cpu_seconds)
+    if cpu:
+        log(f"cpu: {json.dumps(cpu)}")
+
+    ms_per_step = mixed["wall_s"] * 1e3 / args.steps
+    out = {
+        "metric": "solves/s + p50/p99 latency, FP32-LU+FP64-IR vs FP64 LU at residual 1e-10",
+        "value": round(mixed["solves_per_s"], 4),
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 factor + f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"serving stream: {n_req} Ez requests, grids 128-768, Poisson arrivals "
+                               f"{args.rate}/s, 1-4 RHS each, residual {args.tol:g}",
+                   "requests": n_req, "batch": args.batch, "slots": args.slots, "parallelism": "single GPU"},
+        "p50_latency_ms": round(mixed["p50_latency_s"] * 1e3, 3),
+        "p99_latency_ms": round(mixed["p99_latency_s"] * 1e3, 3),
+        "p50_service_ms": round(mixed["p50_service_s"] * 1e3, 3),
+        "p99_service_ms": round(mixed["p99_service_s"] * 1e3, 3),
+        "max_rel_residual": mixed["max_rel_residual"], "fallbacks": mixed["fallbacks"],
+        "failed": mixed["failed"], "mean_sweeps": mixed["mean_sweeps"],
+        "roofline": {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "roofline_detail": {k: roof[k] for k in ("kernel", "workload", "avg_launch_us", "factor_ms")},
+        "cpu_baseline": cpu,
+    }
+    if fp64:
+        out["fp64_path"] = {
+            "solves_per_s": round(fp64["solves_per_s"], 4),
+            "p50_latency_ms": round(fp64["p50_latency_s"] * 1e3, 3),
+            "p99_latency_ms": round(fp64["p99_latency_s"] * 1e3, 3),
+            "p50_service_ms": round(fp64["p50_service_s"] * 1e3, 3),
+            "p99_service_ms": round(fp64["p99_service_s"] * 1e3, 3),
+            "max_rel_residual": fp64["max_rel_residual"], "failed": fp64["failed"],
+            "mean_sweeps": fp64["mean_sweeps"]}
+        out["mixed_over_fp64"] = round(mixed["solves_per_s"] / fp64["solves_per_s"], 4)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/fdfd_hip.h"
 
@@ -229,7 +230,6 @@ template <typename T> struct FJob {
   int *status;       // set to 1 on a non-finite / zero pivot
   int m, n, np, nt;  // blocks, block size, padded block size, tiles per edge
   int k, p;          // step performed by this launch; k < 0: initial pivot
-  int h, ngroups;    // tile rows per workgroup, row groups
 };
 template <typename T> struct FBatch {
   int njobs;
@@ -274,11 +274,13 @@ __device__ inline void x_elem(const FJob<T> &J, int k, int p, const T *src, int 
 }
 
 template <typename T> struct Lds {
-  T Pr[TB][TBP], Pi[TB][TBP];  // pivot inverse
-  T Rr[TB][TBP], Ri[TB][TBP];  // P * X[p,J]
-  T Ar[TB][TBP], Ai[TB][TBP];  // X[I,p], X[p,J] staging, inversion scratch
+  T Pr[TB][TBP], Pi[TB][TBP];  // pivot inverse; inversion buffer 0
+  T Rr[TB][TBP], Ri[TB][TBP];  // P * X[p,J];   inversion buffer 1
+  T Ar[TB][TBP], Ai[TB][TBP];  // X[p,J]; tile to invert / inverse
+  T Br[TB][TBP], Bi[TB][TBP];  // X[I,p]
+  T mag[2][TB];                // pivot candidates (double-buffered)
   int perm[TB];
-  int piv;
+  int colmap[TB];
 };
 
 template <typename T>
@@ -307,73 +309,95 @@ __device__ inline void cgemm_quad(typename Mfma<T>::v4 &cr, typename Mfma<T>::v4
   }
 }
 
-// In-place inverse of the TB x TB tile (Ar, Ai) by Gauss-Jordan with
-// partial pivoting inside the tile; column swaps undone at the end.
+// Inverse of the TB x TB tile in (Ar, Ai), in place, by Gauss-Jordan with
+// partial pivoting inside the tile. Thread t keeps row t/8, columns
+// 4(t%8)..+3 in registers; each pivot step publishes the tile to one of two
+// LDS buffers (Pr/Pi, Rr/Ri, alternating), so one barrier per pivot orders
+// the exchange. The row interchanges become one permuted write-back.
 template <typename T>
 __device__ void tile_inverse(Lds<T> &L, int *status) {
-  const int tid = threadIdx.x;
+  static_assert(TB * TB == 4 * NT, "4 elements per thread");
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int i = tid >> 3, c0 = (tid & 7) * 4;
+  T ar[4], ai[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { ar[q] = L.Ar[i][c0 + q]; ai[q] = L.Ai[i][c0 + q]; }
   bool bad = false;
   for (int j = 0; j < TB; ++j) {
-    if (tid < 64) {
-      T mag = T(-1);
-      int idx = tid;
-      if (tid >= j && tid < TB) mag = fabs(L.Ar[tid][j]) + fabs(L.Ai[tid][j]);
-      for (int off = 32; off >= 1; off >>= 1) {
-        T om = __shfl_xor(mag, off);
-        int oi = __shfl_xor(idx, off);
-        if (om > mag || (om == mag && oi < idx)) { mag = om; idx = oi; }
-      }
-      if (tid == 0) { L.piv = idx; L.perm[j] = idx; }
+    T (*Br)[TBP] = (j & 1) ? L.Rr : L.Pr;
+    T (*Bi)[TBP] = (j & 1) ? L.Ri : L.Pi;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { Br[i][c0 + q] = ar[q]; Bi[i][c0 + q] = ai[q]; }
+    if (j >= c0 && j < c0 + 4) {
+      const int q = j - c0;
+      L.mag[j & 1][i] = (i >= j) ? fabs(ar[q]) + fabs(ai[q]) : T(-1);
     }
     __syncthreads();
-    const int piv = L.piv;
-    if (piv != j && tid < TB) {
-      T t = L.Ar[j][tid]; L.Ar[j][tid] = L.Ar[piv][tid]; L.Ar[piv][tid] = t;
-      t = L.Ai[j][tid]; L.Ai[j][tid] = L.Ai[piv][tid]; L.Ai[piv][tid] = t;
+    // argmax over the column (ties to the lower row), in every wave
+    T m = (lane < TB) ? L.mag[j & 1][lane] : T(-2);
+    if (!(m == m)) m = T(-1);  // a NaN candidate is never chosen
+    int idx = lane;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const T om = __shfl_xor(m, off);
+      const int oi = __shfl_xor(idx, off);
+      if (om > m || (om == m && oi < idx)) { m = om; idx = oi; }
     }
-    __syncthreads();
-    const T pr = L.Ar[j][j], pi = L.Ai[j][j];
-    if (!(fabs(pr) + fabs(pi) > T(0)) || !isfinite(pr) || !isfinite(pi)) bad = true;
+    const int piv = (idx >= j && idx < TB) ? idx : j;
+    if (tid == 0) L.perm[j] = piv;
+    const int er = (i == j) ? piv : (i == piv ? j : i);  // row content after the swap
+    if (er != i) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) { ar[q] = Br[er][c0 + q]; ai[q] = Bi[er][c0 + q]; }
+    }
+    const T pr = Br[piv][j], pim = Bi[piv][j];
+    if (!(fabs(pr) + fabs(pim) > T(0)) || !isfinite(pr) || !isfinite(pim)) bad = true;
     T dr, di;
-    crecip<T>(pr, pi, dr, di);
-    T nr[TB * TB / NT], ni[TB * TB / NT];
+    crecip<T>(pr, pim, dr, di);
+    if (i == j) {
 #pragma unroll
-    for (int q = 0; q < TB * TB / NT; ++q) {
-      const int e = tid + NT * q, i = e / TB, c = e % TB;
-      if (i == j) {
-        if (c == j) { nr[q] = dr; ni[q] = di; }
+      for (int q = 0; q < 4; ++q) {
+        if (c0 + q == j) { ar[q] = dr; ai[q] = di; }
         else {
-          const T ar = L.Ar[j][c], ai = L.Ai[j][c];
-          nr[q] = ar * dr - ai * di; ni[q] = ar * di + ai * dr;
+          const T xr = ar[q], xi = ai[q];
+          ar[q] = xr * dr - xi * di;
+          ai[q] = xr * di + xi * dr;
         }
-      } else {
-        const T fr = L.Ar[i][j], fi = L.Ai[i][j];
-        const T fdr = fr * dr - fi * di, fdi = fr * di + fi * dr;  // f * d
-        if (c == j) { nr[q] = -fdr; ni[q] = -fdi; }
+      }
+    } else {
+      const T fr = Br[er][j], fi = Bi[er][j];
+      const T gr = fr * dr - fi * di, gi = fr * di + fi * dr;  // f * d
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (c0 + q == j) { ar[q] = -gr; ai[q] = -gi; }
         else {
-          const T ar = L.Ar[j][c], ai = L.Ai[j][c];
-          nr[q] = L.Ar[i][c] - (fdr * ar - fdi * ai);
-          ni[q] = L.Ai[i][c] - (fdr * ai + fdi * ar);
+          const T hr = Br[piv][c0 + q], hi = Bi[piv][c0 + q];
+          ar[q] -= gr * hr - gi * hi;
+          ai[q] -= gr * hi + gi * hr;
         }
       }
     }
-    __syncthreads();
+  }
+  // TB is even, so buffer 0 (Pr/Pi) was last read two steps ago
 #pragma unroll
-    for (int q = 0; q < TB * TB / NT; ++q) {
-      const int e = tid + NT * q, i = e / TB, c = e % TB;
-      L.Ar[i][c] = nr[q]; L.Ai[i][c] = ni[q];
+  for (int q = 0; q < 4; ++q) { L.Pr[i][c0 + q] = ar[q]; L.Pi[i][c0 + q] = ai[q]; }
+  if (tid == 0) {
+    for (int c = 0; c < TB; ++c) L.colmap[c] = c;
+    for (int j = TB - 1; j >= 0; --j) {
+      const int pj = L.perm[j], t = L.colmap[j];
+      L.colmap[j] = L.colmap[pj];
+      L.colmap[pj] = t;
     }
-    __syncthreads();
   }
-  for (int j = TB - 1; j >= 0; --j) {
-    const int pj = L.perm[j];
-    if (pj != j && tid < TB) {
-      T t = L.Ar[tid][j]; L.Ar[tid][j] = L.Ar[tid][pj]; L.Ar[tid][pj] = t;
-      t = L.Ai[tid][j]; L.Ai[tid][j] = L.Ai[tid][pj]; L.Ai[tid][pj] = t;
-    }
-    __syncthreads();
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int src = L.colmap[c0 + q];
+    L.Ar[i][c0 + q] = L.Pr[i][src];
+    L.Ai[i][c0 + q] = L.Pi[i][src];
   }
-  if (bad && tid == 0) *status = 1;
+  if (bad) *status = 1;
+  __syncthreads();
 }
 
 template <typename T>
@@ -411,7 +435,14 @@ __device__ inline void schur_next_00(const FJob<T> &J, int k, Lds<T> &L) {
   __syncthreads();
 }
 
-// One Gauss-Jordan step (k, p) for every job of the batch.
+// One Gauss-Jordan step (k, p) for every job of the batch; one output tile
+// (I, J) per workgroup:
+//   I == p, J == p : P                     (P = inverse of X[p,p])
+//   I == p, J != p : P X[p,J]
+//   I != p, J == p : -X[I,p] P
+//   otherwise      : X[I,J] - X[I,p] (P X[p,J])
+// The workgroup owning tile (p+1, p+1) also inverts its result for the next
+// step; at the last step the owner of (0,0) forms and inverts S_{k+1}[0,0].
 template <typename T>
 __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
   __shared__ Lds<T> L;
@@ -434,8 +465,7 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     return;
   }
   const int k = J.k, p = J.p;
-  const int tj = local % nt, grp = local / nt;
-  const int i0 = grp * J.h, i1 = min(nt, i0 + J.h);
+  const int tj = local % nt, ti = local / nt;
   T *Gk = J.G + (size_t)k * 2 * np2;
   const T *src = (p == 0) ? nullptr : (((nt - p) % 2 == 0) ? Gk : J.W);
   T *dst = ((nt - 1 - p) % 2 == 0) ? Gk : J.W;
@@ -443,74 +473,69 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
   const T *Pslot = J.Pinv + (size_t)(step & 1) * 2 * TB * TB;
   T *Pnext = J.Pinv + (size_t)((step + 1) & 1) * 2 * TB * TB;
 
+  // every global read of this tile's update, issued together
   for (int e = tid; e < TB * TB; e += NT) {
     L.Pr[e / TB][e % TB] = Pslot[e];
     L.Pi[e / TB][e % TB] = Pslot[TB * TB + e];
   }
-  if (tj != p) {
-    load_tile(J, k, p, src, p, tj, L.Ar, L.Ai);  // X[p, J]
-    __syncthreads();
-    typename Mfma<T>::v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-    cgemm_quad<T>(cr, ci, L.Pr, L.Pi, L.Ar, L.Ai, qi, qj, lane, T(1));
-    __syncthreads();
+  if (tj != p) load_tile(J, k, p, src, p, tj, L.Ar, L.Ai);   // X[p, J]
+  if (ti != p) load_tile(J, k, p, src, ti, p, L.Br, L.Bi);   // X[I, p]
+  typename Mfma<T>::v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  if (ti != p && tj != p) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
-      L.Rr[a][b] = cr[t];
-      L.Ri[a][b] = ci[t];
+      T xr, xi;
+      x_elem(J, k, p, src, ti * TB + a, tj * TB + b, xr, xi);
+      cr[t] = xr;
+      ci[t] = xi;
     }
   }
   __syncthreads();
-
-  for (int ti = i0; ti < i1; ++ti) {
-    typename Mfma<T>::v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-    if (ti == p) {
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
-        cr[t] = (tj == p) ? L.Pr[a][b] : L.Rr[a][b];
-        ci[t] = (tj == p) ? L.Pi[a][b] : L.Ri[a][b];
-      }
-    } else {
-      load_tile(J, k, p, src, ti, p, L.Ar, L.Ai);  // X[I, p]
-      if (tj != p) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
-          T xr, xi;
-          x_elem(J, k, p, src, ti * TB + a, tj * TB + b, xr, xi);
-          cr[t] = xr;
-          ci[t] = xi;
-        }
-      }
-      __syncthreads();
-      if (tj == p) cgemm_quad<T>(cr, ci, L.Ar, L.Ai, L.Pr, L.Pi, qi, qj, lane, T(-1));
-      else cgemm_quad<T>(cr, ci, L.Ar, L.Ai, L.Rr, L.Ri, qi, qj, lane, T(-1));
-    }
-    // store Y[I, J]
+  if (tj != p) {  // R = P X[p, J]
+    typename Mfma<T>::v4 rr = {0, 0, 0, 0}, ri = {0, 0, 0, 0};
+    cgemm_quad<T>(rr, ri, L.Pr, L.Pi, L.Ar, L.Ai, qi, qj, lane, T(1));
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
-      const size_t idx = (size_t)(ti * TB + a) * J.np + tj * TB + b;
-      dst[idx] = cr[t];
-      dst[np2 + idx] = ci[t];
+      L.Rr[a][b] = rr[t];
+      L.Ri[a][b] = ri[t];
     }
-    const bool next_pivot = (p + 1 < nt) && ti == p + 1 && tj == p + 1;
-    const bool next_block = (p == nt - 1) && ti == 0 && tj == 0 && k + 1 < J.m;
-    __syncthreads();  // Ar/Ai free again
-    if (next_pivot || next_block) {
+    if (ti == p) { cr = rr; ci = ri; }
+    __syncthreads();
+  }
+  if (ti == p && tj == p) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
-        L.Ar[a][b] = cr[t];
-        L.Ai[a][b] = ci[t];
-      }
-      __syncthreads();
-      if (next_block) schur_next_00(J, k, L);
-      tile_inverse(L, J.status);
-      store_pinv(L, Pnext);
-      __syncthreads();
+    for (int t = 0; t < 4; ++t) {
+      const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+      cr[t] = L.Pr[a][b];
+      ci[t] = L.Pi[a][b];
     }
+  } else if (ti != p) {
+    if (tj == p) cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Pr, L.Pi, qi, qj, lane, T(-1));
+    else cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Rr, L.Ri, qi, qj, lane, T(-1));
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+    const size_t idx = (size_t)(ti * TB + a) * J.np + tj * TB + b;
+    dst[idx] = cr[t];
+    dst[np2 + idx] = ci[t];
+  }
+  const bool next_pivot = (p + 1 < nt) && ti == p + 1 && tj == p + 1;
+  const bool next_block = (p == nt - 1) && ti == 0 && tj == 0 && k + 1 < J.m;
+  if (next_pivot || next_block) {
+    __syncthreads();  // all reads of Ar/Ai, Pr/Pi, Rr/Ri are done
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+      L.Ar[a][b] = cr[t];
+      L.Ai[a][b] = ci[t];
+    }
+    __syncthreads();
+    if (next_block) schur_next_00(J, k, L);
+    tile_inverse(L, J.status);
+    store_pinv(L, Pnext);
   }
 }
 
@@ -637,6 +662,143 @@ __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const
       }
     }
     __syncthreads();
+  }
+}
+
+// Persistent sweeps: one launch per direction. Workgroup w owns rows
+// [w*4*SRW, (w+1)*4*SRW) of every block; lane l of a wave covers columns
+// l, l+64, ... of its rows. Blocks are chained through a counter: step s
+// waits until all W workgroups have finished step s-1. Each workgroup loads
+// its rows of G_k before it waits, so HBM latency overlaps the
+// synchronization. A wait that exceeds SPIN_CAP polls raises the abort flag;
+// every later wait then returns at once, so the grid always drains.
+constexpr int SRW = 2;              // rows per wave
+constexpr int MAXCL = 768 / 64;     // columns per lane at the largest block
+constexpr int MAXRHS = 8;           // right-hand sides per pass
+constexpr unsigned SPIN_CAP = 1u << 21;
+
+struct SweepSync {
+  unsigned cnt[2];   // forward, backward step counters
+  unsigned abort;
+  unsigned pad;
+};
+
+__device__ inline void sweep_wait(SweepSync *sy, int dir, unsigned target) {
+  if (threadIdx.x == 0) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(&sy->cnt[dir], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(&sy->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (++spins > SPIN_CAP) {
+        __hip_atomic_store(&sy->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const double *__restrict__ op,
+                                               const double *__restrict__ r, T *y, double *__restrict__ x,
+                                               int nx, int ny, int np, int nrhs, int dir, SweepSync *sy) {
+  extern __shared__ unsigned char smem_raw[];
+  T *vr = reinterpret_cast<T *>(smem_raw);   // [nrhs][np]
+  T *vi = vr + (size_t)nrhs * np;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const size_t np2 = (size_t)np * np, M = (size_t)nx * ny;
+  const int W = gridDim.x;
+  const int row0 = (blockIdx.x * 4 + wv) * SRW;
+  for (int s = 0; s < nx; ++s) {
+    const int k = dir == 0 ? s : nx - 1 - s;
+    const T *Gk = G + (size_t)k * 2 * np2;
+    const bool coupled = dir == 0 ? k > 0 : k + 1 < nx;
+    // 1. this workgroup's rows of G_k (independent of the previous block)
+    T gr[SRW][MAXCL], gi[SRW][MAXCL];
+#pragma unroll
+    for (int rr = 0; rr < SRW; ++rr) {
+      const int a = row0 + rr;
+#pragma unroll
+      for (int cc = 0; cc < MAXCL; ++cc) {
+        const int c = lane + 64 * cc;
+        const bool ok = a < np && c < np && (dir == 0 || coupled);
+        gr[rr][cc] = ok ? Gk[(size_t)a * np + c] : T(0);
+        gi[rr][cc] = ok ? Gk[np2 + (size_t)a * np + c] : T(0);
+      }
+    }
+    // 2. the part of v that does not depend on the previous block
+    for (int e = tid; e < nrhs * np; e += NT) {
+      const int q = e / np, c = e % np;
+      T pr = 0, pi = 0;
+      if (dir == 0 && c < ny) {
+        cd v = ld_c(r, (size_t)q * M + (size_t)k * ny + c);
+        pr = (T)v.r; pi = (T)v.i;
+      }
+      vr[e] = pr;
+      vi[e] = pi;
+    }
+    // 3. wait for the previous block, then fold it into v
+    if (s > 0) sweep_wait(sy, dir, (unsigned)(W * s));
+    else __syncthreads();
+    if (coupled) {
+      const int kp = dir == 0 ? k - 1 : k + 1;
+      const int pl = dir == 0 ? FDFD_OP_XM : FDFD_OP_XP;
+      for (int e = tid; e < nrhs * np; e += NT) {
+        const int q = e / np, c = e % np;
+        if (c < ny) {
+          const T *yp = y + 2 * (((size_t)q * nx + kp) * np + c);
+          const cd cpl = ld_c(op, pl * M + (size_t)k * ny + c);
+          const cd t = cmul(cpl, cd{(double)yp[0], (double)yp[1]});
+          if (dir == 0) { vr[e] = (T)((double)vr[e] - t.r); vi[e] = (T)((double)vi[e] - t.i); }
+          else { vr[e] = (T)t.r; vi[e] = (T)t.i; }
+        }
+      }
+    }
+    __syncthreads();
+    // 4. row dot products on this workgroup's rows
+#pragma unroll
+    for (int rr = 0; rr < SRW; ++rr) {
+      const int a = row0 + rr;
+      for (int q = 0; q < nrhs; ++q) {
+        T sr = 0, si = 0;
+        if (dir == 0 || coupled) {
+#pragma unroll
+          for (int cc = 0; cc < MAXCL; ++cc) {
+            const int c = lane + 64 * cc;
+            if (c < np) {
+              const T br = vr[(size_t)q * np + c], bi = vi[(size_t)q * np + c];
+              sr += gr[rr][cc] * br - gi[rr][cc] * bi;
+              si += gr[rr][cc] * bi + gi[rr][cc] * br;
+            }
+          }
+          for (int off = 32; off >= 1; off >>= 1) {
+            sr += __shfl_xor(sr, off);
+            si += __shfl_xor(si, off);
+          }
+        }
+        if (lane == 0 && a < np) {
+          T *yk = y + 2 * (((size_t)q * nx + k) * np + a);
+          if (dir == 0) {
+            yk[0] = sr;
+            yk[1] = si;
+          } else {
+            const T dr = yk[0] - sr, di = yk[1] - si;
+            yk[0] = dr;
+            yk[1] = di;
+            if (a < ny) {
+              double *xo = x + 2 * ((size_t)q * M + (size_t)k * ny + a);
+              xo[0] += (double)dr;
+              xo[1] += (double)di;
+            }
+          }
+        }
+      }
+    }
+    // 5. publish this block's rows
+    if (lane == 0) __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_fetch_add(&sy->cnt[dir], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -792,7 +954,7 @@ FactorWork factor_work_layout(const fdfd_grid *g, int precision, void *work) {
 }
 
 struct SolveWork {
-  double *r; void *y; double *part; double *norms;
+  double *r; void *y; double *part; double *norms; SweepSync *sync;
 };
 SolveWork solve_work_layout(const fdfd_grid *g, int precision, int nrhs, void *work) {
   const size_t M = (size_t)g->nx * g->ny, np = pad_n(g->ny), es = elem_size(precision);
@@ -802,7 +964,8 @@ SolveWork solve_work_layout(const fdfd_grid *g, int precision, int nrhs, void *w
   sw.r = (double *)(base + off); off += align_up(16 * M * nrhs, 256);
   sw.y = base + off;             off += align_up(2 * es * (size_t)g->nx * np * nrhs, 256);
   sw.part = (double *)(base + off); off += align_up(8 * (size_t)NORM_PARTS * nrhs * 2, 256);
-  sw.norms = (double *)(base + off);
+  sw.norms = (double *)(base + off); off += align_up(8 * 4 * (size_t)nrhs, 256);
+  sw.sync = (SweepSync *)(base + off);
   return sw;
 }
 
@@ -825,13 +988,11 @@ int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, 
   FJob<T> &J = b.job[0];
   J.op = op; J.G = (T *)factor; J.W = (T *)fw.W; J.Pinv = (T *)fw.Pinv; J.status = fw.status;
   J.m = m; J.n = n; J.np = np; J.nt = nt;
-  J.h = 1;
-  J.ngroups = nt;
   J.k = -1; J.p = 0;
   b.wg_start[0] = 0; b.wg_start[1] = 1;
   int rc = launch_step(b, s);
   if (rc) return rc;
-  b.wg_start[1] = nt * J.ngroups;
+  b.wg_start[1] = nt * nt;
   for (int k = 0; k < m; ++k) {
     for (int p = 0; p < nt; ++p) {
       J.k = k; J.p = p;
@@ -843,15 +1004,41 @@ int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, 
 }
 
 template <typename T>
-int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const double *r,
-               double *x, int nrhs, void *y, hipStream_t s) {
+int sweep_blockwise(const fdfd_grid *g, const double *op, const void *factor, const double *r,
+                    double *x, int nrhs, void *y, hipStream_t s) {
   const int nx = g->nx, ny = g->ny, np = pad_n(ny);
   const dim3 grid((np + 3) / 4);
-  for (int k = 0; k < nx; ++k) {
+  for (int k = 0; k < nx; ++k)
     hipLaunchKernelGGL(k_sweep_fwd<T>, grid, dim3(NT), 0, s, (const T *)factor, op, r, (T *)y, nx, ny, np, k, nrhs);
-  }
-  for (int k = nx - 1; k >= 0; --k) {
+  for (int k = nx - 1; k >= 0; --k)
     hipLaunchKernelGGL(k_sweep_bwd<T>, grid, dim3(NT), 0, s, (const T *)factor, op, (T *)y, x, nx, ny, np, k, nrhs);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+// x += F^-1 r with the persistent sweep kernels (RHS in passes of MAXRHS);
+// blockwise = true uses one launch per block instead.
+template <typename T>
+int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const double *r,
+               double *x, int nrhs, const SolveWork &sw, hipStream_t s, bool blockwise) {
+  if (blockwise) return sweep_blockwise<T>(g, op, factor, r, x, nrhs, sw.y, s);
+  const int nx = g->nx, ny = g->ny, np = pad_n(ny);
+  const int W = (np + 4 * SRW - 1) / (4 * SRW);
+  const size_t M = (size_t)nx * ny;
+  static std::once_flag lds_attr;
+  std::call_once(lds_attr, [] {
+    (void)hipFuncSetAttribute((const void *)k_psweep<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(2 * MAXRHS * 768 * sizeof(T)));
+  });
+  for (int q0 = 0; q0 < nrhs; q0 += MAXRHS) {
+    const int nr = std::min(MAXRHS, nrhs - q0);
+    T *yq = (T *)sw.y + (size_t)2 * q0 * nx * np;
+    const size_t lds = (size_t)2 * nr * np * sizeof(T);
+    HIP_TRY(hipMemsetAsync(sw.sync, 0, sizeof(SweepSync), s));
+    hipLaunchKernelGGL(k_psweep<T>, dim3(W), dim3(NT), lds, s, (const T *)factor, op, r + 2 * M * q0, yq,
+                       (double *)nullptr, nx, ny, np, nr, 0, sw.sync);
+    hipLaunchKernelGGL(k_psweep<T>, dim3(W), dim3(NT), lds, s, (const T *)factor, op, (const double *)nullptr,
+                       yq, x + 2 * M * q0, nx, ny, np, nr, 1, sw.sync);
   }
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
@@ -894,7 +1081,8 @@ size_t fdfd_solve_workspace_bytes(const fdfd_grid *g, int precision, int nrhs) {
   if (check_grid(g) || check_prec(precision) || nrhs < 1) return 0;
   const size_t M = (size_t)g->nx * g->ny, np = pad_n(g->ny), es = elem_size(precision);
   return align_up(16 * M * nrhs, 256) + align_up(2 * es * (size_t)g->nx * np * nrhs, 256) +
-         align_up(8 * (size_t)NORM_PARTS * nrhs * 2, 256) + align_up(8 * 4 * (size_t)nrhs, 256);
+         align_up(8 * (size_t)NORM_PARTS * nrhs * 2, 256) + align_up(8 * 4 * (size_t)nrhs, 256) +
+         align_up(sizeof(SweepSync), 256);
 }
 
 int fdfd_assemble(const fdfd_grid *g, const double *eps, double *op, hipStream_t stream) {
@@ -951,8 +1139,14 @@ int fdfd_sweep(const fdfd_grid *g, const double *op, const void *factor, int pre
   if (int rc = check_prec(precision)) return rc;
   if (!op || !factor || !r || !x || !work || nrhs < 1) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_sweep");
   SolveWork sw = solve_work_layout(g, precision, nrhs, work);
-  return precision == FDFD_PREC_FP64 ? sweep_impl<double>(g, op, factor, r, x, nrhs, sw.y, stream)
-                                     : sweep_impl<float>(g, op, factor, r, x, nrhs, sw.y, stream);
+  int rc = precision == FDFD_PREC_FP64 ? sweep_impl<double>(g, op, factor, r, x, nrhs, sw, stream, false)
+                                       : sweep_impl<float>(g, op, factor, r, x, nrhs, sw, stream, false);
+  if (rc) return rc;
+  unsigned ab = 0;
+  HIP_TRY(hipMemcpyAsync(&ab, &sw.sync->abort, sizeof ab, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (ab) return fail(FDFD_ERR_HIP, "sweep workgroups were not co-resident; x is invalid");
+  return FDFD_OK;
 }
 
 int fdfd_norms(const fdfd_grid *g, const double *v, int nrhs, void *work, double *out,
@@ -984,69 +1178,72 @@ int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int pre
     return fail(FDFD_ERR_ARG, "bad arguments to fdfd_solve");
   SolveWork sw = solve_work_layout(g, precision, nrhs, work);
   const size_t M = (size_t)g->nx * g->ny;
+  const dim3 sgrid((g->ny + NT - 1) / NT, g->nx, nrhs);
   fdfd_solve_stats st;
   memset(&st, 0, sizeof st);
-  std::vector<double> nb(nrhs), nr(nrhs), prev(nrhs, INFINITY);
-  // ||b|| (in the caller's system)
+  std::vector<double> nb(nrhs), nr(nrhs), rel(nrhs, 0.0);
+  // ||b|| in the caller's system
   if (int rc = norms_async(g, b, nrhs, 0, sw.part, sw.norms, stream)) return rc;
   HIP_TRY(hipMemcpyAsync(nb.data(), sw.norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, stream));
-  HIP_TRY(hipMemsetAsync(x, 0, 16 * M * nrhs, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   bool all_zero = true;
   for (int q = 0; q < nrhs; ++q) all_zero = all_zero && nb[q] == 0.0;
   if (all_zero) {  // linalg.py:128-129
+    HIP_TRY(hipMemsetAsync(x, 0, 16 * M * nrhs, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
     st.zero_rhs = 1;
     st.converged = 1;
     if (stats) *stats = st;
     if (rel_res) for (int q = 0; q < nrhs; ++q) rel_res[q] = 0;
     return FDFD_OK;
   }
-  // transpose: A^T = D A D^-1, solve A x' = D^-1 b, x = D x'
-  double *bb = sw.r;  // current residual, starts as the (scaled) rhs
-  HIP_TRY(hipMemcpyAsync(bb, b, 16 * M * nrhs, hipMemcpyDeviceToDevice, stream));
-  if (transpose) {
-    if (int rc = fdfd_stretch_scale(g, bb, nrhs, 1, stream)) return rc;
-  }
-  // the scaled rhs is needed for every residual: keep it in x's slot? No -
-  // residual r = b' - A x' is formed from b' kept in the caller's b when not
-  // transposed; for transpose a second copy lives after the iterate.
-  std::vector<double> rel(nrhs, 0.0);
-  int stall_count = 0;
+  // transpose: A^T = D A D^-1, so A^T x = b is A x' = D^-1 b with x = D x'
+  double *rr = sw.r;  // residual of the iterate, in the solved system
   int rc = FDFD_OK;
-  // b' must persist: use x as the iterate and rebuild r each iteration.
-  // For transpose, b' = D^-1 b is recomputed from b into r before each residual.
-  for (int it = 1; it <= max_iter; ++it) {
-    rc = precision == FDFD_PREC_FP64 ? sweep_impl<double>(g, op, factor, bb, x, nrhs, sw.y, stream)
-                                     : sweep_impl<float>(g, op, factor, bb, x, nrhs, sw.y, stream);
-    if (rc) return rc;
-    st.iterations = it;
-    if (transpose) {
-      HIP_TRY(hipMemcpyAsync(bb, b, 16 * M * nrhs, hipMemcpyDeviceToDevice, stream));
-      if ((rc = fdfd_stretch_scale(g, bb, nrhs, 1, stream))) return rc;
-      hipLaunchKernelGGL(k_stencil, dim3((g->ny + NT - 1) / NT, g->nx, nrhs), dim3(NT), 0, stream,
-                         g->nx, g->ny, op, (const double *)x, (const double *)bb, bb);
-    } else {
-      hipLaunchKernelGGL(k_stencil, dim3((g->ny + NT - 1) / NT, g->nx, nrhs), dim3(NT), 0, stream,
-                         g->nx, g->ny, op, (const double *)x, b, bb);
+  bool blockwise = false;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    memset(&st, 0, sizeof st);
+    HIP_TRY(hipMemsetAsync(x, 0, 16 * M * nrhs, stream));
+    HIP_TRY(hipMemcpyAsync(rr, b, 16 * M * nrhs, hipMemcpyDeviceToDevice, stream));
+    if (transpose && (rc = fdfd_stretch_scale(g, rr, nrhs, 1, stream))) return rc;
+    std::vector<double> prev(nrhs, INFINITY);
+    int stall_count = 0;
+    bool aborted = false;
+    for (int it = 1; it <= max_iter; ++it) {
+      rc = precision == FDFD_PREC_FP64 ? sweep_impl<double>(g, op, factor, rr, x, nrhs, sw, stream, blockwise)
+                                       : sweep_impl<float>(g, op, factor, rr, x, nrhs, sw, stream, blockwise);
+      if (rc) return rc;
+      st.iterations = it;
+      // r = b' - A x, with b' = b or D^-1 b
+      HIP_TRY(hipMemcpyAsync(rr, b, 16 * M * nrhs, hipMemcpyDeviceToDevice, stream));
+      if (transpose && (rc = fdfd_stretch_scale(g, rr, nrhs, 1, stream))) return rc;
+      hipLaunchKernelGGL(k_stencil, sgrid, dim3(NT), 0, stream, g->nx, g->ny, op, (const double *)x,
+                         (const double *)rr, rr);
+      HIP_TRY(hipGetLastError());
+      // transposed: ||b - A^T x|| = ||D (b' - A x')||
+      if ((rc = norms_async(g, rr, nrhs, transpose, sw.part, sw.norms, stream))) return rc;
+      unsigned ab = 0;
+      HIP_TRY(hipMemcpyAsync(nr.data(), sw.norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipMemcpyAsync(&ab, &sw.sync->abort, sizeof ab, hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+      if (ab && !blockwise) { aborted = true; break; }
+      double worst = 0;
+      bool contracting = false;
+      for (int q = 0; q < nrhs; ++q) {
+        rel[q] = nb[q] > 0 ? sqrt(nr[q] / nb[q]) : 0.0;
+        if (!std::isfinite(rel[q])) rel[q] = INFINITY;
+        worst = std::max(worst, rel[q]);
+        if (rel[q] > tol && rel[q] < 0.7 * prev[q]) contracting = true;
+      }
+      st.max_rel_residual = worst;
+      if (worst <= tol) { st.converged = 1; break; }
+      if (!std::isfinite(worst)) { st.stalled = 1; break; }
+      stall_count = contracting ? 0 : stall_count + 1;
+      if (stall_count >= 2) { st.stalled = 1; break; }
+      for (int q = 0; q < nrhs; ++q) prev[q] = rel[q];
     }
-    HIP_TRY(hipGetLastError());
-    if ((rc = norms_async(g, bb, nrhs, transpose, sw.part, sw.norms, stream))) return rc;
-    HIP_TRY(hipMemcpyAsync(nr.data(), sw.norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
-    double worst = 0;
-    bool contracting = false;
-    for (int q = 0; q < nrhs; ++q) {
-      rel[q] = nb[q] > 0 ? sqrt(nr[q] / nb[q]) : 0.0;
-      if (!std::isfinite(rel[q])) rel[q] = INFINITY;
-      worst = std::max(worst, rel[q]);
-      if (rel[q] > tol && rel[q] < 0.7 * prev[q]) contracting = true;
-    }
-    st.max_rel_residual = worst;
-    if (worst <= tol) { st.converged = 1; break; }
-    if (!std::isfinite(worst)) { st.stalled = 1; break; }
-    stall_count = contracting ? 0 : stall_count + 1;
-    if (stall_count >= 2) { st.stalled = 1; break; }
-    for (int q = 0; q < nrhs; ++q) prev[q] = rel[q];
+    if (!aborted) break;
+    blockwise = true;  // persistent sweep could not run co-resident: redo with per-block launches
   }
   if (transpose) {
     if ((rc = fdfd_stretch_scale(g, x, nrhs, 0, stream))) return rc;

@@ -114,7 +114,7 @@ class ServingEngine:
     """
 
     def __init__(self, device="cuda:0", precision="c64", slots=4, memory_budget=None,
-                 tol=1e-10, max_iter=80, cache_bytes=0, keep_fields=False):
+                 tol=1e-10, max_iter=80, cache_bytes=0, keep_fields=False, serve_fn=None):
         self.device = torch.device(device)
         self.precision = precision
         self.slots = slots
@@ -126,7 +126,13 @@ class ServingEngine:
         self.max_iter = max_iter
         self.cache = FactorCache(cache_bytes)
         self.keep_fields = keep_fields
-        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(slots)]
+        # serve_fn(request, stream) -> (fields, SolveReport, reused) replaces the
+        # device work (scheduler tests run it on the CPU)
+        self.serve_fn = serve_fn or self._serve
+        self.streams = ([torch.cuda.Stream(device=self.device) for _ in range(slots)]
+                        if serve_fn is None else [None] * slots)
+        self.max_in_flight = 0
+        self.max_bytes_in_flight = 0
 
     # ------------------------------------------------------------ one request
     def _serve(self, req, stream):
@@ -164,7 +170,7 @@ class ServingEngine:
             err = ""
             x, rep, reused = None, None, False
             try:
-                x, rep, reused = self._serve(req, self.streams[slot])
+                x, rep, reused = self.serve_fn(req, self.streams[slot])
             except Exception as e:  # reported per request, the loop continues
                 err = f"{type(e).__name__}: {e}"
             done = now()
@@ -200,6 +206,8 @@ class ServingEngine:
                 slot = state["free_slots"].pop(0)
                 state["used"] += need
                 state["next"] += 1
+                self.max_in_flight = max(self.max_in_flight, self.slots - len(state["free_slots"]))
+                self.max_bytes_in_flight = max(self.max_bytes_in_flight, state["used"])
                 t = threading.Thread(target=worker, args=(req, slot, need, now()), daemon=True)
                 threads.append(t)
                 t.start()

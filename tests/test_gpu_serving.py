@@ -1,0 +1,50 @@
+"""The serving loop on the GPU: each result is the same alone and in the mix."""
+
+import numpy as np
+import pytest
+import torch
+
+from fdfdpy_amd import configs as C
+from fdfdpy_amd.serving import Request, ServingEngine, summarize
+from oracle import construct_A
+
+pytestmark = pytest.mark.gpu
+
+
+def _reqs():
+    sizes = [128, 192, 128, 256, 192, 128]
+    return [Request(i, C.c3_request(40 + i, n=sizes[i]), 0.0) for i in range(len(sizes))]
+
+
+@pytest.mark.parametrize("precision", ["c64", "fp64"])
+def test_alone_equals_in_mix(precision):
+    reqs = _reqs()
+    mix = ServingEngine("cuda:0", precision=precision, slots=4, keep_fields=True).run(reqs)
+    assert all(not r.error for r in mix), [r.error for r in mix]
+    for target in (0, 3):
+        alone = ServingEngine("cuda:0", precision=precision, slots=1, keep_fields=True).run([reqs[target]])[0]
+        a = alone.fields.cpu().numpy()
+        m = mix[target].fields.cpu().numpy()
+        # same operator, same kernels, same order of operations: identical bits
+        assert np.array_equal(a, m), np.abs(a - m).max()
+    for r, req in zip(mix, reqs):
+        p = req.problem
+        N = p.shape
+        A, _ = construct_A(p.omega, [0, N[0] * p.dl], [0, N[1] * p.dl], p.eps, p.npml, p.pol, p.L0)
+        x = r.fields.cpu().numpy()
+        b = p.rhs()
+        for q in range(b.shape[0]):
+            res = np.linalg.norm(A @ x[q].reshape(-1) - b[q].reshape(-1)) / np.linalg.norm(b[q])
+            assert res <= 1.5e-10
+
+
+def test_memory_budget_and_factor_cache():
+    reqs = _reqs()[:3]
+    dup = Request(99, reqs[1].problem, 0.0)       # same operator as request 1
+    eng = ServingEngine("cuda:0", precision="c64", slots=2, keep_fields=True, cache_bytes=1 << 32)
+    res = eng.run(reqs)
+    res2 = eng.run([dup])
+    assert res2[0].factor_reused and not res[1].factor_reused
+    assert np.array_equal(res2[0].fields.cpu().numpy(), res[1].fields.cpu().numpy())
+    s = summarize(res, max(r.done for r in res))
+    assert s["failed"] == 0 and s["max_rel_residual"] <= 1e-10
