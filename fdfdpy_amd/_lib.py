@@ -69,6 +69,11 @@ SIGNATURES = {
     "fdfd_kerr_update": (c_int, [_G, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fdfd_fields": (c_int, [_G, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "fdfd_selftest_mfma": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fdfd_engine_create": (c_void_p, [c_int]),
+    "fdfd_engine_destroy": (c_int, [c_void_p]),
+    "fdfd_engine_submit": (c_int, [c_void_p, _G, c_void_p, c_void_p, c_int, c_void_p, POINTER(ctypes.c_uint64)]),
+    "fdfd_engine_wait": (c_int, [c_void_p, ctypes.c_uint64]),
+    "fdfd_engine_launches": (ctypes.c_uint64, [c_void_p]),
 }
 
 _lib = None

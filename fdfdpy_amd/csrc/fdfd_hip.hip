@@ -19,6 +19,11 @@
 #include <vector>
 #include <algorithm>
 #include <mutex>
+#include <condition_variable>
+#include <thread>
+#include <deque>
+#include <map>
+#include <memory>
 
 #include "../../include/fdfd_hip.h"
 
@@ -1054,6 +1059,124 @@ int norms_async(const fdfd_grid *g, const double *v, int nrhs, int weighted, dou
   return FDFD_OK;
 }
 
+// ------------------------------------------------------- batching engine
+// A scheduler thread owns one stream and advances every submitted
+// factorization by one Gauss-Jordan step per launch: the jobs in flight
+// share each k_gj_step launch (up to MAXJ per launch, one precision per
+// launch), new jobs join at the next launch and finished jobs leave, so
+// factorizations of different sizes progress together.
+struct EngineJob {
+  fdfd_grid g;
+  const double *op;
+  void *factor;
+  void *work;
+  int precision;
+  int m, n, np, nt;
+  int k = -1, p = 0;          // next step; k = -1: initial pivot
+  bool launched_all = false;  // every step is on the stream
+  hipEvent_t done = nullptr;
+  int rc = FDFD_OK;
+};
+
+}  // namespace
+
+struct fdfd_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<EngineJob>> pending;
+  std::vector<std::shared_ptr<EngineJob>> active;
+  std::map<uint64_t, std::shared_ptr<EngineJob>> jobs;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  int last_prec = 0;
+  uint64_t launches = 0;
+};
+
+namespace {
+
+template <typename T>
+void fill_job(FJob<T> &J, const EngineJob &e) {
+  FactorWork fw = factor_work_layout(&e.g, e.precision, e.work);
+  J.op = e.op; J.G = (T *)e.factor; J.W = (T *)fw.W; J.Pinv = (T *)fw.Pinv; J.status = fw.status;
+  J.m = e.m; J.n = e.n; J.np = e.np; J.nt = e.nt; J.k = e.k; J.p = e.p;
+}
+
+// One batched launch over up to MAXJ jobs of precision T; advances their steps.
+template <typename T>
+int engine_launch(fdfd_engine *E, std::vector<std::shared_ptr<EngineJob>> &sel) {
+  FBatch<T> b;
+  memset(&b, 0, sizeof b);
+  b.njobs = (int)sel.size();
+  int wg = 0;
+  for (int j = 0; j < b.njobs; ++j) {
+    fill_job<T>(b.job[j], *sel[j]);
+    b.wg_start[j] = wg;
+    wg += sel[j]->k < 0 ? 1 : sel[j]->nt * sel[j]->nt;
+  }
+  b.wg_start[b.njobs] = wg;
+  hipLaunchKernelGGL(k_gj_step<T>, dim3(wg), dim3(NT), 0, E->stream, b);
+  hipError_t err = hipGetLastError();
+  for (auto &e : sel) {
+    if (err != hipSuccess) { e->rc = FDFD_ERR_HIP; e->launched_all = true; continue; }
+    if (e->k < 0) { e->k = 0; e->p = 0; continue; }
+    if (++e->p == e->nt) { e->p = 0; ++e->k; }
+    if (e->k == e->m) {
+      e->launched_all = true;
+      if (hipEventRecord(e->done, E->stream) != hipSuccess) e->rc = FDFD_ERR_HIP;
+    }
+  }
+  ++E->launches;
+  return err == hipSuccess ? FDFD_OK : FDFD_ERR_HIP;
+}
+
+void engine_loop(fdfd_engine *E) {
+  (void)hipSetDevice(E->device);
+  for (;;) {
+    std::vector<std::shared_ptr<EngineJob>> sel;
+    int prec = 0;
+    {
+      std::unique_lock<std::mutex> lk(E->mu);
+      E->cv_work.wait(lk, [E] { return E->stop || !E->pending.empty() || !E->active.empty(); });
+      while (!E->pending.empty()) {
+        auto e = E->pending.front();
+        E->pending.pop_front();
+        FactorWork fw = factor_work_layout(&e->g, e->precision, e->work);
+        if (hipMemsetAsync(fw.status, 0, sizeof(int), E->stream) != hipSuccess) e->rc = FDFD_ERR_HIP;
+        E->active.push_back(e);
+      }
+      if (E->active.empty()) {
+        if (E->stop) return;
+        continue;
+      }
+      // alternate precisions when both are in flight
+      bool has[2] = {false, false};
+      for (auto &e : E->active) has[e->precision] = true;
+      prec = (has[0] && has[1]) ? 1 - E->last_prec : (has[0] ? 0 : 1);
+      E->last_prec = prec;
+      for (auto &e : E->active)
+        if (e->precision == prec && (int)sel.size() < MAXJ) sel.push_back(e);
+    }
+    if (prec == FDFD_PREC_FP64) engine_launch<double>(E, sel);
+    else engine_launch<float>(E, sel);
+    {
+      std::lock_guard<std::mutex> lk(E->mu);
+      bool any = false;
+      for (auto it = E->active.begin(); it != E->active.end();) {
+        if ((*it)->launched_all) { it = E->active.erase(it); any = true; }
+        else ++it;
+      }
+      // round robin: jobs that just ran go to the back
+      if (!any && E->active.size() > (size_t)MAXJ)
+        std::rotate(E->active.begin(), E->active.begin() + std::min(sel.size(), E->active.size()),
+                    E->active.end());
+      if (any) E->cv_done.notify_all();
+    }
+  }
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -1289,6 +1412,79 @@ int fdfd_fields(const fdfd_grid *g, const double *eps, const double *x, double *
                      eps, x, f1, f2, averaging);
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
+}
+
+fdfd_engine *fdfd_engine_create(int device) {
+  auto *E = new (std::nothrow) fdfd_engine();
+  if (!E) { fail(FDFD_ERR_HIP, "out of host memory"); return nullptr; }
+  E->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
+    fail(FDFD_ERR_HIP, "cannot create the engine stream on device %d", device);
+    delete E;
+    return nullptr;
+  }
+  E->worker = std::thread(engine_loop, E);
+  return E;
+}
+
+int fdfd_engine_destroy(fdfd_engine *E) {
+  if (!E) return FDFD_OK;
+  {
+    std::lock_guard<std::mutex> lk(E->mu);
+    E->stop = true;
+  }
+  E->cv_work.notify_all();
+  E->worker.join();
+  (void)hipStreamSynchronize(E->stream);
+  for (auto &kv : E->jobs)
+    if (kv.second->done) (void)hipEventDestroy(kv.second->done);
+  (void)hipStreamDestroy(E->stream);
+  delete E;
+  return FDFD_OK;
+}
+
+int fdfd_engine_submit(fdfd_engine *E, const fdfd_grid *g, const double *op, void *factor, int precision,
+                       void *work, uint64_t *ticket) {
+  if (!E || !ticket) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_engine_submit");
+  if (int rc = check_grid(g)) return rc;
+  if (int rc = check_prec(precision)) return rc;
+  if (!op || !factor || !work) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_engine_submit");
+  auto e = std::make_shared<EngineJob>();
+  e->g = *g; e->op = op; e->factor = factor; e->work = work; e->precision = precision;
+  e->m = g->nx; e->n = g->ny; e->np = pad_n(g->ny); e->nt = e->np / TB;
+  HIP_TRY(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
+  {
+    std::lock_guard<std::mutex> lk(E->mu);
+    *ticket = E->next_ticket++;
+    E->jobs[*ticket] = e;
+    E->pending.push_back(e);
+  }
+  E->cv_work.notify_all();
+  return FDFD_OK;
+}
+
+int fdfd_engine_wait(fdfd_engine *E, uint64_t ticket) {
+  if (!E) return fail(FDFD_ERR_ARG, "engine is NULL");
+  std::shared_ptr<EngineJob> e;
+  {
+    std::unique_lock<std::mutex> lk(E->mu);
+    auto it = E->jobs.find(ticket);
+    if (it == E->jobs.end()) return fail(FDFD_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    e = it->second;
+    E->cv_done.wait(lk, [&] { return e->launched_all; });
+    E->jobs.erase(it);
+  }
+  hipError_t err = hipEventSynchronize(e->done);
+  (void)hipEventDestroy(e->done);
+  if (err != hipSuccess) return fail(FDFD_ERR_HIP, "engine factor: %s", hipGetErrorString(err));
+  if (e->rc) return fail(e->rc, "engine factor launch failed");
+  return fdfd_factor_status(&e->g, e->precision, e->work, E->stream);
+}
+
+uint64_t fdfd_engine_launches(fdfd_engine *E) {
+  if (!E) return 0;
+  std::lock_guard<std::mutex> lk(E->mu);
+  return E->launches;
 }
 
 int fdfd_selftest_mfma(int precision, const double *a, const double *b, double *d, hipStream_t stream) {

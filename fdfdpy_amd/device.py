@@ -77,6 +77,47 @@ class SolveReport:
     history: list = field(default_factory=list)
 
 
+class FactorEngine:
+    """Step-level batching of factorizations (fdfd_engine_* in the C ABI).
+
+    Factorizations submitted from any thread share the engine's Gauss-Jordan
+    step launches; wait() blocks until one is complete.
+    """
+
+    def __init__(self, device):
+        self.lib = L.lib()
+        self.device = torch.device(device)
+        idx = self.device.index if self.device.index is not None else torch.cuda.current_device()
+        h = self.lib.fdfd_engine_create(idx)
+        if not h:
+            raise L.FdfdError(L.FDFD_ERR_HIP, self.lib.fdfd_last_error().decode())
+        self.handle = ctypes.c_void_p(h)
+
+    def submit(self, grid, op, fac, precision, work):
+        t = ctypes.c_uint64(0)
+        L.check(self.lib.fdfd_engine_submit(self.handle, ctypes.byref(grid), _vp(op), _vp(fac),
+                                            L.PREC[precision], _vp(work), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        L.check(self.lib.fdfd_engine_wait(self.handle, ctypes.c_uint64(ticket)))
+
+    @property
+    def launches(self):
+        return int(self.lib.fdfd_engine_launches(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.lib.fdfd_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceOperator:
     """Assembled operator + resident factors for one (grid, omega, eps)."""
 
@@ -107,6 +148,7 @@ class DeviceOperator:
         self.version = 0         # bumped whenever the operator's diagonal changes
         self.factors = {}        # precision -> (uint8 tensor, operator version)
         self.factor_seconds = {}
+        self.engine = None       # FactorEngine shared with other operators, if any
 
     # -- sizes ----------------------------------------------------------
     def factor_bytes(self, precision):
@@ -162,8 +204,12 @@ class DeviceOperator:
         work = torch.empty(self.lib.fdfd_factor_workspace_bytes(g, p), dtype=torch.uint8, device=self.device)
         s = _stream(self.device)
         t0 = time.perf_counter()
-        L.check(self.lib.fdfd_factor(g, _vp(self.op), _vp(fac), p, _vp(work), s))
-        L.check(self.lib.fdfd_factor_status(g, p, _vp(work), s))
+        if self.engine is not None:
+            torch.cuda.current_stream(self.device).synchronize()   # operator complete
+            self.engine.wait(self.engine.submit(self.grid, self.op, fac, precision, work))
+        else:
+            L.check(self.lib.fdfd_factor(g, _vp(self.op), _vp(fac), p, _vp(work), s))
+            L.check(self.lib.fdfd_factor_status(g, p, _vp(work), s))
         self.factor_seconds[precision] = time.perf_counter() - t0
         self.factors[precision] = (fac, self.version)
         return fac

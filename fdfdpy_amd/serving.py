@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .device import DeviceOperator
+from .device import DeviceOperator, FactorEngine
 
 
 @dataclass
@@ -110,11 +110,15 @@ class ServingEngine:
     'fp64'. slots: requests in flight at once (one host thread and one HIP
     stream each). memory_budget: device bytes for in-flight requests
     (default: 85% of the free memory at construction). cache_bytes: device
-    bytes for factors kept after their request completes.
+    bytes for factors kept after their request completes. batch_factors:
+    the factorizations of in-flight requests advance together in shared
+    Gauss-Jordan step launches (the native engine) instead of one launch
+    sequence per request stream.
     """
 
     def __init__(self, device="cuda:0", precision="c64", slots=4, memory_budget=None,
-                 tol=1e-10, max_iter=80, cache_bytes=0, keep_fields=False, serve_fn=None):
+                 tol=1e-10, max_iter=80, cache_bytes=0, keep_fields=False, serve_fn=None,
+                 batch_factors=True):
         self.device = torch.device(device)
         self.precision = precision
         self.slots = slots
@@ -133,6 +137,8 @@ class ServingEngine:
                         if serve_fn is None else [None] * slots)
         self.max_in_flight = 0
         self.max_bytes_in_flight = 0
+        # factorizations of in-flight requests share step launches
+        self.factor_engine = FactorEngine(self.device) if (batch_factors and serve_fn is None) else None
 
     # ------------------------------------------------------------ one request
     def _serve(self, req, stream):
@@ -145,6 +151,7 @@ class ServingEngine:
                 reused = True
             else:
                 op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, self.device)
+                op.engine = self.factor_engine
                 op.set_eps(p.eps)
             b = torch.as_tensor(p.rhs(), device=self.device)
             x, rep = op.solve(b, precision=self.precision, tol=self.tol, max_iter=self.max_iter)

@@ -152,6 +152,23 @@ int fdfd_kerr_update(const fdfd_grid *g, double *op, const double *base_c,
 int fdfd_fields(const fdfd_grid *g, const double *eps, const double *x, double *f1, double *f2,
                 int nrhs, int averaging, hipStream_t stream);
 
+/* Step-level batching engine (no reference counterpart; serves the request
+ * loop). A scheduler thread with its own stream advances every submitted
+ * factorization by one Gauss-Jordan step per launch; the factorizations in
+ * flight (up to 8 per launch, grouped by precision) share each launch and
+ * new ones join at the next launch. Arguments of fdfd_engine_submit are as
+ * for fdfd_factor and must stay valid until fdfd_engine_wait returns; the
+ * operator must be complete on the device before submission.
+ * fdfd_engine_wait blocks until the factor is complete and returns its
+ * numeric status (as fdfd_factor_status). */
+typedef struct fdfd_engine fdfd_engine;
+fdfd_engine *fdfd_engine_create(int device);
+int fdfd_engine_destroy(fdfd_engine *engine);
+int fdfd_engine_submit(fdfd_engine *engine, const fdfd_grid *g, const double *op, void *factor,
+                       int precision, void *work, uint64_t *ticket);
+int fdfd_engine_wait(fdfd_engine *engine, uint64_t ticket);
+uint64_t fdfd_engine_launches(fdfd_engine *engine);
+
 /* Check of the MFMA operand layouts the factor kernels rely on (no reference
  * counterpart): D = A (16x4) * B (4x16) with the 16x16x4 f64 (precision 0)
  * or f32 (precision 1) instruction; a, b, d are row-major device arrays of

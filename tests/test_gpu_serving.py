@@ -48,3 +48,31 @@ def test_memory_budget_and_factor_cache():
     assert np.array_equal(res2[0].fields.cpu().numpy(), res[1].fields.cpu().numpy())
     s = summarize(res, max(r.done for r in res))
     assert s["failed"] == 0 and s["max_rel_residual"] <= 1e-10
+
+
+def test_engine_factors_match_direct_factors():
+    """Factorizations batched in shared step launches equal the one-by-one ones."""
+    import threading
+    from fdfdpy_amd.device import DeviceOperator, FactorEngine
+    eng = FactorEngine("cuda:0")
+    probs = [C.c3_request(60 + i, n=n) for i, n in enumerate((128, 192, 256, 128))]
+    precs = ["c64", "fp64", "c64", "c64"]
+    ops_a, ops_b = [], []
+    for p in probs:
+        for ops, e in ((ops_a, None), (ops_b, eng)):
+            op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, "cuda:0")
+            op.set_eps(p.eps)
+            op.engine = e
+            ops.append(op)
+    for op, pr in zip(ops_a, precs):
+        op.factor(pr)
+    threads = [threading.Thread(target=op.factor, args=(pr,)) for op, pr in zip(ops_b, precs)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    torch.cuda.synchronize()
+    for a, b, pr in zip(ops_a, ops_b, precs):
+        assert torch.equal(a.factors[pr][0], b.factors[pr][0])
+    assert eng.launches < sum(p.shape[0] * ((p.shape[1] + 31) // 32) for p in probs)
+    eng.close()
