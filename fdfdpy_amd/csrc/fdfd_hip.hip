@@ -314,13 +314,91 @@ __device__ inline void cgemm_quad(typename Mfma<T>::v4 &cr, typename Mfma<T>::v4
   }
 }
 
+// 1/(zr + i zi) from hardware reciprocals with one Newton step each (no IEEE
+// division sequences on the critical path); scaled by max(|zr|, |zi|) so
+// the squared modulus cannot overflow.
+__device__ inline float rcp_nr(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  return r * (2.0f - x * r);
+}
+__device__ inline double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = r * (2.0 - x * r);
+  return r * (2.0 - x * r);
+}
+template <typename T>
+__device__ inline void fast_crecip(T zr, T zi, T &outr, T &outi) {
+  const T sc = fmax(fabs(zr), fabs(zi));
+  const T rs = rcp_nr(sc);
+  const T ar = zr * rs, ai = zi * rs;
+  const T rd = rcp_nr(ar * ar + ai * ai) * rs;
+  outr = ar * rd;
+  outi = -ai * rd;
+}
+
 // Inverse of the TB x TB tile in (Ar, Ai), in place, by Gauss-Jordan with
-// partial pivoting inside the tile. Thread t keeps row t/8, columns
+// diagonal pivots (the default; the tile is a diagonal block of a Schur
+// complement of the PML-damped operator). Thread t keeps row t/8, columns
+// 4(t%8)..+3 in registers and publishes them each step to one of two LDS
+// buffers (Pr/Pi, Rr/Ri), so one barrier per pivot orders the exchange.
+template <typename T>
+__device__ void tile_inverse_diag(Lds<T> &L, int *status) {
+  static_assert(TB * TB == 4 * NT, "4 elements per thread");
+  const int tid = threadIdx.x;
+  const int i = tid >> 3, c0 = (tid & 7) * 4;
+  T ar[4], ai[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { ar[q] = L.Ar[i][c0 + q]; ai[q] = L.Ai[i][c0 + q]; }
+  bool bad = false;
+#pragma unroll 2
+  for (int j = 0; j < TB; ++j) {
+    T (*Br)[TBP] = (j & 1) ? L.Rr : L.Pr;
+    T (*Bi)[TBP] = (j & 1) ? L.Ri : L.Pi;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { Br[i][c0 + q] = ar[q]; Bi[i][c0 + q] = ai[q]; }
+    __syncthreads();
+    const T pr = Br[j][j], pim = Bi[j][j];
+    if (!(fabs(pr) + fabs(pim) > T(0)) || !isfinite(pr) || !isfinite(pim)) bad = true;
+    T dr, di;
+    fast_crecip<T>(pr, pim, dr, di);
+    if (i == j) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (c0 + q == j) { ar[q] = dr; ai[q] = di; }
+        else {
+          const T xr = ar[q], xi = ai[q];
+          ar[q] = xr * dr - xi * di;
+          ai[q] = xr * di + xi * dr;
+        }
+      }
+    } else {
+      const T fr = Br[i][j], fi = Bi[i][j];
+      const T gr = fr * dr - fi * di, gi = fr * di + fi * dr;  // f * d
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (c0 + q == j) { ar[q] = -gr; ai[q] = -gi; }
+        else {
+          const T hr = Br[j][c0 + q], hi = Bi[j][c0 + q];
+          ar[q] -= gr * hr - gi * hi;
+          ai[q] -= gr * hi + gi * hr;
+        }
+      }
+    }
+  }
+  // (Ar, Ai) is not read inside the loop
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { L.Ar[i][c0 + q] = ar[q]; L.Ai[i][c0 + q] = ai[q]; }
+  if (bad) *status = 1;
+  __syncthreads();
+}
+
+// Inverse of the TB x TB tile in (Ar, Ai), in place, by Gauss-Jordan with
+// partial pivoting inside the tile (the fallback factorization). Thread t keeps row t/8, columns
 // 4(t%8)..+3 in registers; each pivot step publishes the tile to one of two
 // LDS buffers (Pr/Pi, Rr/Ri, alternating), so one barrier per pivot orders
 // the exchange. The row interchanges become one permuted write-back.
 template <typename T>
-__device__ void tile_inverse(Lds<T> &L, int *status) {
+__device__ void tile_inverse_pivoted(Lds<T> &L, int *status) {
   static_assert(TB * TB == 4 * NT, "4 elements per thread");
   const int tid = threadIdx.x, lane = tid & 63;
   const int i = tid >> 3, c0 = (tid & 7) * 4;
@@ -405,6 +483,12 @@ __device__ void tile_inverse(Lds<T> &L, int *status) {
   __syncthreads();
 }
 
+template <typename T, bool PIV>
+__device__ inline void tile_inverse(Lds<T> &L, int *status) {
+  if (PIV) tile_inverse_pivoted(L, status);
+  else tile_inverse_diag(L, status);
+}
+
 template <typename T>
 __device__ inline void store_pinv(const Lds<T> &L, T *slot) {
   for (int e = threadIdx.x; e < TB * TB; e += NT) {
@@ -448,7 +532,7 @@ __device__ inline void schur_next_00(const FJob<T> &J, int k, Lds<T> &L) {
 //   otherwise      : X[I,J] - X[I,p] (P X[p,J])
 // The workgroup owning tile (p+1, p+1) also inverts its result for the next
 // step; at the last step the owner of (0,0) forms and inverts S_{k+1}[0,0].
-template <typename T>
+template <typename T, bool PIV>
 __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
   __shared__ Lds<T> L;
   int jb = 0;
@@ -464,7 +548,7 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     if (local == 0) {
       load_tile(J, 0, 0, (const T *)nullptr, 0, 0, L.Ar, L.Ai);
       __syncthreads();
-      tile_inverse(L, J.status);
+      tile_inverse<T, PIV>(L, J.status);
       store_pinv(L, J.Pinv);  // slot of global step 0
     }
     return;
@@ -539,7 +623,7 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     }
     __syncthreads();
     if (next_block) schur_next_00(J, k, L);
-    tile_inverse(L, J.status);
+    tile_inverse<T, PIV>(L, J.status);
     store_pinv(L, Pnext);
   }
 }
@@ -677,6 +761,10 @@ __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const
 // its rows of G_k before it waits, so HBM latency overlaps the
 // synchronization. A wait that exceeds SPIN_CAP polls raises the abort flag;
 // every later wait then returns at once, so the grid always drains.
+// The vectors passed between workgroups (y_k, d_k) are written and read with
+// agent-scope relaxed atomics, which are coherent across the XCDs' L2 caches
+// by themselves; the counter needs only workgroup-scope ordering around it,
+// so no step writes back or invalidates a whole L2.
 constexpr int SRW = 2;              // rows per wave
 constexpr int MAXCL = 768 / 64;     // columns per lane at the largest block
 constexpr int MAXRHS = 8;           // right-hand sides per pass
@@ -699,9 +787,17 @@ __device__ inline void sweep_wait(SweepSync *sy, int dir, unsigned target) {
       }
       __builtin_amdgcn_s_sleep(2);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
+}
+
+template <typename T>
+__device__ inline T ld_xcd(const T *p) {  // coherent across workgroups on any XCD
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline void st_xcd(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T>
@@ -754,7 +850,7 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
         if (c < ny) {
           const T *yp = y + 2 * (((size_t)q * nx + kp) * np + c);
           const cd cpl = ld_c(op, pl * M + (size_t)k * ny + c);
-          const cd t = cmul(cpl, cd{(double)yp[0], (double)yp[1]});
+          const cd t = cmul(cpl, cd{(double)ld_xcd(yp), (double)ld_xcd(yp + 1)});
           if (dir == 0) { vr[e] = (T)((double)vr[e] - t.r); vi[e] = (T)((double)vi[e] - t.i); }
           else { vr[e] = (T)t.r; vi[e] = (T)t.i; }
         }
@@ -785,12 +881,12 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
         if (lane == 0 && a < np) {
           T *yk = y + 2 * (((size_t)q * nx + k) * np + a);
           if (dir == 0) {
-            yk[0] = sr;
-            yk[1] = si;
+            st_xcd(yk, sr);
+            st_xcd(yk + 1, si);
           } else {
-            const T dr = yk[0] - sr, di = yk[1] - si;
-            yk[0] = dr;
-            yk[1] = di;
+            const T dr = yk[0] - sr, di = yk[1] - si;  // own rows, written by the forward launch
+            st_xcd(yk, dr);
+            st_xcd(yk + 1, di);
             if (a < ny) {
               double *xo = x + 2 * ((size_t)q * M + (size_t)k * ny + a);
               xo[0] += (double)dr;
@@ -800,10 +896,10 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
         }
       }
     }
-    // 5. publish this block's rows
-    if (lane == 0) __threadfence();
+    // 5. publish this block's rows: the barrier waits for every wave's
+    // stores (workgroup-scope release), then one relaxed agent increment
     __syncthreads();
-    if (tid == 0) __hip_atomic_fetch_add(&sy->cnt[dir], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) __hip_atomic_fetch_add(&sy->cnt[dir], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -937,11 +1033,14 @@ int check_grid(const fdfd_grid *g) {
   return FDFD_OK;
 }
 int check_prec(int precision) {
-  if (precision != FDFD_PREC_FP64 && precision != FDFD_PREC_C64)
-    return fail(FDFD_ERR_ARG, "precision must be FDFD_PREC_FP64 or FDFD_PREC_C64, got %d", precision);
+  const int base = precision & 0xff, flags = precision & ~0xff;
+  if ((base != FDFD_PREC_FP64 && base != FDFD_PREC_C64) || (flags & ~FDFD_FACTOR_PIVOT))
+    return fail(FDFD_ERR_ARG, "precision must be FDFD_PREC_FP64 or FDFD_PREC_C64 (optionally | "
+                "FDFD_FACTOR_PIVOT), got %d", precision);
   return FDFD_OK;
 }
-inline size_t elem_size(int precision) { return precision == FDFD_PREC_FP64 ? 8 : 4; }
+inline int base_prec(int precision) { return precision & 0xff; }
+inline size_t elem_size(int precision) { return base_prec(precision) == FDFD_PREC_FP64 ? 8 : 4; }
 
 struct FactorWork {
   void *W; void *Pinv; int *status;
@@ -974,18 +1073,19 @@ SolveWork solve_work_layout(const fdfd_grid *g, int precision, int nrhs, void *w
   return sw;
 }
 
-template <typename T>
+template <typename T, bool PIV>
 int launch_step(const FBatch<T> &b, hipStream_t s) {
   const int grid = b.wg_start[b.njobs];
-  hipLaunchKernelGGL(k_gj_step<T>, dim3(grid), dim3(NT), 0, s, b);
+  hipLaunchKernelGGL((k_gj_step<T, PIV>), dim3(grid), dim3(NT), 0, s, b);
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
 }
 
-template <typename T>
+template <typename T, bool PIV>
 int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, hipStream_t s) {
   const int n = g->ny, np = pad_n(n), nt = np / TB, m = g->nx;
   FactorWork fw = factor_work_layout(g, sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64, work);
+  (void)PIV;
   HIP_TRY(hipMemsetAsync(fw.status, 0, sizeof(int), s));
   FBatch<T> b;
   memset(&b, 0, sizeof b);
@@ -995,13 +1095,13 @@ int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, 
   J.m = m; J.n = n; J.np = np; J.nt = nt;
   J.k = -1; J.p = 0;
   b.wg_start[0] = 0; b.wg_start[1] = 1;
-  int rc = launch_step(b, s);
+  int rc = launch_step<T, PIV>(b, s);
   if (rc) return rc;
   b.wg_start[1] = nt * nt;
   for (int k = 0; k < m; ++k) {
     for (int p = 0; p < nt; ++p) {
       J.k = k; J.p = p;
-      rc = launch_step(b, s);
+      rc = launch_step<T, PIV>(b, s);
       if (rc) return rc;
     }
   }
@@ -1070,7 +1170,8 @@ struct EngineJob {
   const double *op;
   void *factor;
   void *work;
-  int precision;
+  int precision;               // FDFD_PREC_*
+  bool pivot;                  // partial pivoting inside pivot tiles
   int m, n, np, nt;
   int k = -1, p = 0;          // next step; k = -1: initial pivot
   bool launched_all = false;  // every step is on the stream
@@ -1105,7 +1206,7 @@ void fill_job(FJob<T> &J, const EngineJob &e) {
 }
 
 // One batched launch over up to MAXJ jobs of precision T; advances their steps.
-template <typename T>
+template <typename T, bool PIV>
 int engine_launch(fdfd_engine *E, std::vector<std::shared_ptr<EngineJob>> &sel) {
   FBatch<T> b;
   memset(&b, 0, sizeof b);
@@ -1117,7 +1218,7 @@ int engine_launch(fdfd_engine *E, std::vector<std::shared_ptr<EngineJob>> &sel) 
     wg += sel[j]->k < 0 ? 1 : sel[j]->nt * sel[j]->nt;
   }
   b.wg_start[b.njobs] = wg;
-  hipLaunchKernelGGL(k_gj_step<T>, dim3(wg), dim3(NT), 0, E->stream, b);
+  hipLaunchKernelGGL((k_gj_step<T, PIV>), dim3(wg), dim3(NT), 0, E->stream, b);
   hipError_t err = hipGetLastError();
   for (auto &e : sel) {
     if (err != hipSuccess) { e->rc = FDFD_ERR_HIP; e->launched_all = true; continue; }
@@ -1151,16 +1252,22 @@ void engine_loop(fdfd_engine *E) {
         if (E->stop) return;
         continue;
       }
-      // alternate precisions when both are in flight
-      bool has[2] = {false, false};
-      for (auto &e : E->active) has[e->precision] = true;
-      prec = (has[0] && has[1]) ? 1 - E->last_prec : (has[0] ? 0 : 1);
+      // one launch per kind (precision x pivoting); kinds take turns
+      bool has[4] = {false, false, false, false};
+      for (auto &e : E->active) has[e->precision * 2 + (e->pivot ? 1 : 0)] = true;
+      prec = E->last_prec;
+      for (int t = 1; t <= 4; ++t)
+        if (has[(E->last_prec + t) % 4]) { prec = (E->last_prec + t) % 4; break; }
       E->last_prec = prec;
       for (auto &e : E->active)
-        if (e->precision == prec && (int)sel.size() < MAXJ) sel.push_back(e);
+        if (e->precision * 2 + (e->pivot ? 1 : 0) == prec && (int)sel.size() < MAXJ) sel.push_back(e);
     }
-    if (prec == FDFD_PREC_FP64) engine_launch<double>(E, sel);
-    else engine_launch<float>(E, sel);
+    switch (prec) {
+      case 0: engine_launch<double, false>(E, sel); break;
+      case 1: engine_launch<double, true>(E, sel); break;
+      case 2: engine_launch<float, false>(E, sel); break;
+      default: engine_launch<float, true>(E, sel); break;
+    }
     {
       std::lock_guard<std::mutex> lk(E->mu);
       bool any = false;
@@ -1241,8 +1348,12 @@ int fdfd_factor(const fdfd_grid *g, const double *op, void *factor, int precisio
   if (int rc = check_grid(g)) return rc;
   if (int rc = check_prec(precision)) return rc;
   if (!op || !factor || !work) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_factor");
-  return precision == FDFD_PREC_FP64 ? factor_impl<double>(g, op, factor, work, stream)
-                                     : factor_impl<float>(g, op, factor, work, stream);
+  const bool piv = precision & FDFD_FACTOR_PIVOT;
+  if (base_prec(precision) == FDFD_PREC_FP64)
+    return piv ? factor_impl<double, true>(g, op, factor, work, stream)
+               : factor_impl<double, false>(g, op, factor, work, stream);
+  return piv ? factor_impl<float, true>(g, op, factor, work, stream)
+             : factor_impl<float, false>(g, op, factor, work, stream);
 }
 
 int fdfd_factor_status(const fdfd_grid *g, int precision, void *work, hipStream_t stream) {
@@ -1262,8 +1373,9 @@ int fdfd_sweep(const fdfd_grid *g, const double *op, const void *factor, int pre
   if (int rc = check_prec(precision)) return rc;
   if (!op || !factor || !r || !x || !work || nrhs < 1) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_sweep");
   SolveWork sw = solve_work_layout(g, precision, nrhs, work);
-  int rc = precision == FDFD_PREC_FP64 ? sweep_impl<double>(g, op, factor, r, x, nrhs, sw, stream, false)
-                                       : sweep_impl<float>(g, op, factor, r, x, nrhs, sw, stream, false);
+  int rc = base_prec(precision) == FDFD_PREC_FP64
+               ? sweep_impl<double>(g, op, factor, r, x, nrhs, sw, stream, false)
+               : sweep_impl<float>(g, op, factor, r, x, nrhs, sw, stream, false);
   if (rc) return rc;
   unsigned ab = 0;
   HIP_TRY(hipMemcpyAsync(&ab, &sw.sync->abort, sizeof ab, hipMemcpyDeviceToHost, stream));
@@ -1333,8 +1445,9 @@ int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int pre
     int stall_count = 0;
     bool aborted = false;
     for (int it = 1; it <= max_iter; ++it) {
-      rc = precision == FDFD_PREC_FP64 ? sweep_impl<double>(g, op, factor, rr, x, nrhs, sw, stream, blockwise)
-                                       : sweep_impl<float>(g, op, factor, rr, x, nrhs, sw, stream, blockwise);
+      rc = base_prec(precision) == FDFD_PREC_FP64
+               ? sweep_impl<double>(g, op, factor, rr, x, nrhs, sw, stream, blockwise)
+               : sweep_impl<float>(g, op, factor, rr, x, nrhs, sw, stream, blockwise);
       if (rc) return rc;
       st.iterations = it;
       // r = b' - A x, with b' = b or D^-1 b
@@ -1450,7 +1563,8 @@ int fdfd_engine_submit(fdfd_engine *E, const fdfd_grid *g, const double *op, voi
   if (int rc = check_prec(precision)) return rc;
   if (!op || !factor || !work) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_engine_submit");
   auto e = std::make_shared<EngineJob>();
-  e->g = *g; e->op = op; e->factor = factor; e->work = work; e->precision = precision;
+  e->g = *g; e->op = op; e->factor = factor; e->work = work;
+  e->precision = base_prec(precision); e->pivot = precision & FDFD_FACTOR_PIVOT;
   e->m = g->nx; e->n = g->ny; e->np = pad_n(g->ny); e->nt = e->np / TB;
   HIP_TRY(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
   {

@@ -94,9 +94,10 @@ class FactorEngine:
         self.handle = ctypes.c_void_p(h)
 
     def submit(self, grid, op, fac, precision, work):
+        """precision: an FDFD_PREC_* code, optionally | FACTOR_PIVOT."""
         t = ctypes.c_uint64(0)
         L.check(self.lib.fdfd_engine_submit(self.handle, ctypes.byref(grid), _vp(op), _vp(fac),
-                                            L.PREC[precision], _vp(work), ctypes.byref(t)))
+                                            int(precision), _vp(work), ctypes.byref(t)))
         return t.value
 
     def wait(self, ticket):
@@ -146,7 +147,7 @@ class DeviceOperator:
         self.eps_user = None     # complex128, caller orientation
         self.base_c = None       # linear diagonal plane (for Kerr updates)
         self.version = 0         # bumped whenever the operator's diagonal changes
-        self.factors = {}        # precision -> (uint8 tensor, operator version)
+        self.factors = {}        # precision -> (uint8 tensor, operator version, pivoted)
         self.factor_seconds = {}
         self.engine = None       # FactorEngine shared with other operators, if any
 
@@ -191,27 +192,32 @@ class DeviceOperator:
             self.version = self.linear_version
 
     # -- factor --------------------------------------------------------------
-    def factor(self, precision="fp64", refresh=False):
+    def factor(self, precision="fp64", refresh=False, pivot=False):
         """Resident factor for this precision; built for the current operator
-        when absent or when refresh is set and it belongs to an older one."""
+        when absent, or when refresh is set and the resident one belongs to
+        an older operator or another pivoting mode. pivot=True pivots inside
+        each 32x32 pivot tile (the fallback when the diagonal-pivot factor
+        does not refine)."""
         if precision in self.factors:
-            fac, ver = self.factors[precision]
-            if not refresh or ver == self.version:
+            fac, ver, piv = self.factors[precision]
+            if not refresh or (ver == self.version and piv == pivot):
                 return fac
         g = ctypes.byref(self.grid)
-        p = L.PREC[precision]
-        fac = torch.empty(self.lib.fdfd_factor_bytes(g, p), dtype=torch.uint8, device=self.device)
-        work = torch.empty(self.lib.fdfd_factor_workspace_bytes(g, p), dtype=torch.uint8, device=self.device)
+        p = L.PREC[precision] | (L.FACTOR_PIVOT if pivot else 0)
+        fac = torch.empty(self.lib.fdfd_factor_bytes(g, L.PREC[precision]), dtype=torch.uint8,
+                          device=self.device)
+        work = torch.empty(self.lib.fdfd_factor_workspace_bytes(g, L.PREC[precision]), dtype=torch.uint8,
+                           device=self.device)
         s = _stream(self.device)
         t0 = time.perf_counter()
         if self.engine is not None:
             torch.cuda.current_stream(self.device).synchronize()   # operator complete
-            self.engine.wait(self.engine.submit(self.grid, self.op, fac, precision, work))
+            self.engine.wait(self.engine.submit(self.grid, self.op, fac, p, work))
         else:
             L.check(self.lib.fdfd_factor(g, _vp(self.op), _vp(fac), p, _vp(work), s))
             L.check(self.lib.fdfd_factor_status(g, p, _vp(work), s))
         self.factor_seconds[precision] = time.perf_counter() - t0
-        self.factors[precision] = (fac, self.version)
+        self.factors[precision] = (fac, self.version, pivot)
         return fac
 
     def drop_factor(self, precision=None):
@@ -225,8 +231,9 @@ class DeviceOperator:
               fallback=True):
         """x with A x = b (or A^T x = b), b: (nrhs, Nx, Ny) or (Nx, Ny) complex.
 
-        precision 'c64' factors in complex64 and refines in FP64; when the
-        refinement stalls it falls back to the FP64 factor.
+        precision 'c64' factors in complex64 and refines in FP64. When the
+        refinement stalls the solve falls back to the FP64 factor, and then
+        to the FP64 factor with pivoting inside the pivot tiles.
         """
         squeeze = b.dim() == 2
         bb = b.reshape((1,) + self.shape) if squeeze else b
@@ -234,20 +241,24 @@ class DeviceOperator:
         bs = self.to_solver(bb.to(device=self.device, dtype=torch.complex128))
         x = torch.empty_like(bs)
         rep = self._solve_one(bs, x, nrhs, precision, tol, max_iter, transpose)
-        if not rep.converged and self.factors[precision][1] != self.version:
-            # the resident factor belongs to an earlier diagonal (Born steps):
-            # refactor the current operator and try again
-            self.factor(precision, refresh=True)
-            rep2 = self._solve_one(bs, x, nrhs, precision, tol, max_iter, transpose)
+        attempts = []
+        if self.factors[precision][1] != self.version:
+            # the resident factor belongs to an earlier diagonal (Born steps)
+            attempts.append((precision, False))
+        if precision == "c64" and fallback:
+            attempts.append(("fp64", False))
+        if fallback:
+            # last resort: partial pivoting inside the 32x32 pivot tiles
+            attempts.append(("fp64", True))
+        for prec, piv in attempts:
+            if rep.converged:
+                break
+            self.factor(prec, refresh=True, pivot=piv)
+            rep2 = self._solve_one(bs, x, nrhs, prec, tol, max_iter, transpose)
+            rep2.fell_back = prec != precision or piv
             rep2.factor_seconds += rep.factor_seconds
             rep2.solve_seconds += rep.solve_seconds
             rep = rep2
-        if not rep.converged and precision == "c64" and fallback:
-            rep64 = self._solve_one(bs, x, nrhs, "fp64", tol, max_iter, transpose)
-            rep64.fell_back = True
-            rep64.factor_seconds += rep.factor_seconds
-            rep64.solve_seconds += rep.solve_seconds
-            rep = rep64
         if not rep.converged:
             raise L.NotConverged(L.FDFD_ERR_NOT_CONVERGED,
                                  f"solve did not reach rel. residual {tol:g}: {rep.max_rel_residual:.3e} "

@@ -157,7 +157,7 @@ class ServingEngine:
             x, rep = op.solve(b, precision=self.precision, tol=self.tol, max_iter=self.max_iter)
             stream.synchronize()
         if self.cache.capacity and not reused:
-            nbytes = op.op_bytes() + sum(f.numel() for f, _ in op.factors.values())
+            nbytes = op.op_bytes() + sum(f[0].numel() for f in op.factors.values())
             self.cache.put(key, op, nbytes)
         return x, rep, reused
 

@@ -48,6 +48,10 @@ enum {
 
 enum { FDFD_POL_EZ = 0, FDFD_POL_HZ = 1 };          /* simulation.py:256 */
 enum { FDFD_PREC_FP64 = 0, FDFD_PREC_C64 = 1 };      /* factor precision  */
+/* OR'ed into the precision argument of fdfd_factor / fdfd_engine_submit:
+ * partial pivoting inside each 32x32 pivot tile (slower; the fallback when
+ * the default diagonal-pivot factor does not refine to the tolerance). */
+#define FDFD_FACTOR_PIVOT 0x100
 enum { FDFD_OP_C = 0, FDFD_OP_XM = 1, FDFD_OP_XP = 2, FDFD_OP_YM = 3, FDFD_OP_YP = 4 };
 
 /* Grid and physics parameters; the fields of Simulation.__init__

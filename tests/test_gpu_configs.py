@@ -1,0 +1,85 @@
+"""Configs 4 (adjoint gradient, 300x300) and 5 (Kerr Born, 400x400) at full size."""
+
+import numpy as np
+import pytest
+import torch
+
+from fdfdpy_amd import configs as C
+from fdfdpy_amd.adjoint import FieldSolver
+from fdfdpy_amd.device import DeviceOperator
+from fdfdpy_amd.simulation import Simulation
+from oracle import OracleSimulation
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def test_c4_adjoint_gradient_vs_central_differences():
+    p = C.c4_adjoint()
+    probe = torch.tensor(p.extra['probe'], device=DEV)
+    src = torch.tensor(p.srcs[0], device=DEV)
+    solver = FieldSolver(p.shape, p.omega, p.dl, p.npml, device=DEV)
+    eps = torch.tensor(p.eps, device=DEV, requires_grad=True)
+    Ez = solver(eps, src)
+    J = (Ez.abs() ** 2 * probe).sum()
+    J.backward()
+    grad = eps.grad.cpu().numpy()
+    assert np.all(grad[~p.extra['design']] != 0)    # defined everywhere, checked in the design region
+    h = p.extra['fd_step']
+
+    def objective_gpu(e):
+        with torch.no_grad():
+            x = solver(torch.tensor(e, device=DEV), src)
+            return float((x.abs() ** 2 * probe).sum())
+
+    def objective_oracle(e):
+        s = OracleSimulation(p.omega, e, p.dl, p.npml, 'Ez')
+        s.src = p.srcs[0].copy()
+        _, _, Ez = s.solve_fields()
+        return float(np.sum(np.abs(Ez) ** 2 * p.extra['probe']))
+
+    errs = []
+    for n_pix, (i, j) in enumerate(p.extra['fd_pixels']):
+        ep, em = p.eps.copy(), p.eps.copy()
+        ep[i, j] += h
+        em[i, j] -= h
+        fd = (objective_gpu(ep) - objective_gpu(em)) / (2 * h)
+        errs.append(abs(fd - grad[i, j]) / (abs(grad[i, j]) + 1e-3 * np.abs(grad).max()))
+        if n_pix < 3:   # through the oracle (the reference's own solver path)
+            fdr = (objective_oracle(ep) - objective_oracle(em)) / (2 * h)
+            assert abs(fdr - grad[i, j]) <= 1e-4 * abs(fdr) + 1e-6 * np.abs(grad).max()
+    print(f"c4: max relative FD mismatch over 32 pixels {max(errs):.2e}")
+    assert max(errs) < 1e-4
+
+
+def test_c5_born_full_size():
+    p = C.c5_kerr()
+    amps = p.extra['amps']
+    sim = Simulation(p.omega, p.eps, p.dl, p.npml, 'Ez')
+    sim.add_nl(p.extra['chi3'], p.extra['nl_region'])
+    a_max = float(np.max(amps))
+    steps = []
+    for a in sorted(amps):
+        sim.src = p.srcs[0] * a
+        sim.fields = {k: None for k in sim.fields}
+        Hx, Hy, Ez, conv = sim.solve_fields_nl(solver_nl='born', conv_threshold=1e-10, max_num_iter=50)
+        c = conv.ravel()
+        nsteps = int(np.count_nonzero(c))
+        assert c[nsteps - 1] < 1e-10, f"amp {a}: Born did not converge"
+        steps.append(nsteps)
+        # fixed point of (A + A_nl(E)) E = b on the device operator
+        op = sim.operator
+        enl = op.kerr_update(torch.tensor(Ez, device=DEV), torch.as_tensor(
+            3 * p.extra['chi3'] / 1e-12 * p.extra['nl_region'], device=DEV))
+        r = op.apply(torch.tensor(Ez, device=DEV)) - torch.tensor(sim.src * 1j * p.omega, device=DEV)
+        op.restore_linear()
+        assert (torch.linalg.vector_norm(r) / (p.omega * np.linalg.norm(sim.src))).item() < 1e-8
+        if a == a_max:
+            E_top = Ez
+    # the strongest amplitude against the oracle's Born iteration
+    ref = OracleSimulation(p.omega, p.eps, p.dl, p.npml, 'Ez')
+    ref.src = p.srcs[0] * a_max
+    ref.add_nl(p.extra['chi3'], p.extra['nl_region'])
+    _, _, rE, rconv = ref.born_solve(conv_threshold=1e-10, max_num_iter=50)
+    assert np.linalg.norm(E_top - rE) / np.linalg.norm(rE) <= 1e-9
+    print(f"c5: Born steps per amplitude {steps}")

@@ -184,3 +184,21 @@ def test_adjoint_gradient_matches_oracle():
     assert abs(J.item() - Jr) <= 1e-10 * Jr
     g = e.grad.cpu().numpy()
     assert relerr(g, gr) <= 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "c64"])
+def test_pivoted_tile_factor_matches_oracle(precision):
+    """The fallback factor (partial pivoting inside the pivot tiles)."""
+    rng = np.random.default_rng(8)
+    N = (50, 70)
+    eps = 1 + 11.25 * (rng.random(N) < 0.5)
+    sim = OracleSimulation(OMEGA, eps, 0.02, [9, 9], 'Hz')
+    op = DeviceOperator(N, OMEGA, 0.02, [9, 9], 'Hz', 1e-6, DEV)
+    op.set_eps(eps)
+    op.factor(precision, pivot=True)
+    assert op.factors[precision][2] is True
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    x, rep = op.solve(torch.tensor(b, device=DEV), precision=precision, tol=1e-13)
+    assert rep.converged and not rep.fell_back
+    xref = solver_direct(sim.A, b.reshape(-1)).reshape(N)
+    assert relerr(x.cpu().numpy(), xref) < 1e-10
