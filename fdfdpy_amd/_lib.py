@@ -20,7 +20,8 @@ FDFD_ERR_NUMERIC = 4
 
 POL = {"Ez": 0, "Hz": 1}
 PREC = {"fp64": 0, "c64": 1}
-FACTOR_PIVOT = 0x100   # FDFD_FACTOR_PIVOT
+FACTOR_PIVOT = 0x100     # FDFD_FACTOR_PIVOT
+FACTOR_STEPWISE = 0x200  # FDFD_FACTOR_STEPWISE
 
 
 class FdfdGrid(ctypes.Structure):

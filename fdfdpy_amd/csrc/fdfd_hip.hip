@@ -226,6 +226,39 @@ __global__ void k_selftest_mfma_f32(const double *a, const double *b, double *d)
   for (int t = 0; t < 4; ++t) d[Mfma<float>::crow(l, t) * 16 + l % 16] = c[t];
 }
 
+// ------------------------------------------------ cross-workgroup exchange
+// Data passed between workgroups of one launch is written and read with
+// agent-scope relaxed atomics: coherent across the XCDs' L2 caches by
+// themselves, so a flag needs only workgroup-scope ordering (the barrier's
+// wait for every wave's stores) and no whole-L2 writeback or invalidate.
+// Every wait is bounded: after SPIN_CAP polls it raises an abort flag, and
+// every later wait returns at once, so a grid always drains.
+constexpr unsigned SPIN_CAP = 1u << 21;
+
+template <typename T>
+__device__ inline T ld_xcd(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline void st_xcd(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// thread 0 waits until *flag >= target (or abort); then the whole group
+__device__ inline void wait_flag(const unsigned *flag, unsigned target, unsigned *abort) {
+  if (threadIdx.x == 0) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (++spins > SPIN_CAP) {
+        __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+}
+
 // --------------------------------------------------------- factor: step job
 template <typename T> struct FJob {
   const double *op;  // band storage of this operator
@@ -628,6 +661,171 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
   }
 }
 
+// ------------------------------------------------ persistent factorization
+// One launch factors one operator. Workgroup (I, J) keeps tile (I, J) of the
+// working matrix in its MFMA accumulators for the whole factorization. At
+// step p of block k the owners of row p and column p publish their tiles
+// before updating them, the owner of (p, p) inverts its tile and publishes
+// the inverse, and every other workgroup waits for the three tiles it needs.
+// Flags hold the block number + 1, so they never need resetting within a
+// launch; one grid barrier per block guards the reuse of the exchange
+// buffers. At the end of block k every workgroup writes its tile of G_k and
+// applies the Schur update of block k+1 to its registers.
+template <typename T> struct PJob {
+  const double *op;
+  T *G;            // [m][2][np][np]
+  T *xrow;         // [nt][nt][2][TB*TB]  published X[p, J]
+  T *xcol;         // [nt][nt][2][TB*TB]  published X[I, p]
+  T *pinv;         // [nt][2][TB*TB]      published inverses
+  unsigned *sync;  // [0] abort, [1] barrier, [2, 2+nt) pinv flags, then row / col flags
+  int *status;
+  int m, n, np, nt;
+};
+
+template <typename T>
+__device__ inline void publish_tile(T *dst, const typename Mfma<T>::v4 &cr, const typename Mfma<T>::v4 &ci,
+                                    int qi, int qj, int lane) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int e = (qi * 16 + Mfma<T>::crow(lane, t)) * TB + qj * 16 + (lane & 15);
+    st_xcd(dst + e, cr[t]);
+    st_xcd(dst + TB * TB + e, ci[t]);
+  }
+}
+template <typename T>
+__device__ inline void fetch_tile(const T *src, T (*dr)[TBP], T (*di)[TBP]) {
+  for (int e = threadIdx.x; e < TB * TB; e += NT) {
+    dr[e / TB][e % TB] = ld_xcd(src + e);
+    di[e / TB][e % TB] = ld_xcd(src + TB * TB + e);
+  }
+}
+template <typename T>
+__device__ inline void acc_to_lds(const typename Mfma<T>::v4 &cr, const typename Mfma<T>::v4 &ci,
+                                  T (*dr)[TBP], T (*di)[TBP], int qi, int qj, int lane) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+    dr[a][b] = cr[t];
+    di[a][b] = ci[t];
+  }
+}
+
+template <typename T, bool PIV>
+__global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
+  __shared__ Lds<T> L;
+  const int nt = J.nt;
+  const int ti = blockIdx.x / nt, tj = blockIdx.x % nt;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int qi = w >> 1, qj = w & 1;
+  const size_t np2 = (size_t)J.np * J.np, tsz = 2 * TB * TB;
+  unsigned *abort = J.sync, *barrier = J.sync + 1, *fpinv = J.sync + 2;
+  unsigned *frow = fpinv + nt, *fcol = frow + nt * nt;
+  typedef typename Mfma<T>::v4 v4;
+  v4 cr, ci;
+  // S_0 = T_0
+  FJob<T> X;  // x_elem's view of the operator (p == 0 form)
+  X.op = J.op; X.G = J.G; X.m = J.m; X.n = J.n; X.np = J.np; X.nt = nt;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int a = ti * TB + qi * 16 + Mfma<T>::crow(lane, t), b = tj * TB + qj * 16 + (lane & 15);
+    T xr, xi;
+    x_elem(X, 0, 0, (const T *)nullptr, a, b, xr, xi);
+    cr[t] = xr;
+    ci[t] = xi;
+  }
+  for (int k = 0; k < J.m; ++k) {
+    const unsigned tag = (unsigned)k + 1;
+    if (k > 0) {  // S_k = T_k - l_k G_{k-1} u_{k-1}, elementwise on this tile
+      const int n = J.n;
+      const size_t M = (size_t)J.m * n;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int a = ti * TB + qi * 16 + Mfma<T>::crow(lane, t), b = tj * TB + qj * 16 + (lane & 15);
+        double vr, vi;
+        if (a >= n || b >= n) { vr = (a == b) ? 1.0 : 0.0; vi = 0.0; }
+        else {
+          const size_t pa = (size_t)k * n + a;
+          vr = 0; vi = 0;
+          if (a == b) { cd c = ld_c(J.op, FDFD_OP_C * M + pa); vr += c.r; vi += c.i; }
+          if (b == (a + 1) % n) { cd c = ld_c(J.op, FDFD_OP_YP * M + pa); vr += c.r; vi += c.i; }
+          if (b == (a + n - 1) % n) { cd c = ld_c(J.op, FDFD_OP_YM * M + pa); vr += c.r; vi += c.i; }
+          cd sc = cmul(cmul(ld_c(J.op, FDFD_OP_XM * M + pa), cd{(double)cr[t], (double)ci[t]}),
+                       ld_c(J.op, FDFD_OP_XP * M + (size_t)(k - 1) * n + b));
+          vr -= sc.r; vi -= sc.i;
+        }
+        cr[t] = (T)vr;
+        ci[t] = (T)vi;
+      }
+    }
+    for (int p = 0; p < nt; ++p) {
+      T *prow = J.xrow + ((size_t)p * nt + tj) * tsz;
+      T *pcol = J.xcol + ((size_t)p * nt + ti) * tsz;
+      T *ppiv = J.pinv + (size_t)p * tsz;
+      if (ti == p && tj == p) {
+        acc_to_lds<T>(cr, ci, L.Ar, L.Ai, qi, qj, lane);
+        __syncthreads();
+        tile_inverse<T, PIV>(L, J.status);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+          cr[t] = L.Ar[a][b];
+          ci[t] = L.Ai[a][b];
+        }
+        publish_tile<T>(ppiv, cr, ci, qi, qj, lane);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(fpinv + p, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else if (ti == p) {  // X[p, J] <- P X[p, J]
+        publish_tile<T>(prow, cr, ci, qi, qj, lane);
+        acc_to_lds<T>(cr, ci, L.Ar, L.Ai, qi, qj, lane);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(frow + p * nt + tj, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        wait_flag(fpinv + p, tag, abort);
+        fetch_tile<T>(ppiv, L.Pr, L.Pi);
+        __syncthreads();
+        cr = v4{0, 0, 0, 0};
+        ci = v4{0, 0, 0, 0};
+        cgemm_quad<T>(cr, ci, L.Pr, L.Pi, L.Ar, L.Ai, qi, qj, lane, T(1));
+      } else if (tj == p) {  // X[I, p] <- -X[I, p] P
+        publish_tile<T>(pcol, cr, ci, qi, qj, lane);
+        acc_to_lds<T>(cr, ci, L.Br, L.Bi, qi, qj, lane);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(fcol + p * nt + ti, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        wait_flag(fpinv + p, tag, abort);
+        fetch_tile<T>(ppiv, L.Pr, L.Pi);
+        __syncthreads();
+        cr = v4{0, 0, 0, 0};
+        ci = v4{0, 0, 0, 0};
+        cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Pr, L.Pi, qi, qj, lane, T(-1));
+      } else {  // X[I, J] -= X[I, p] (P X[p, J])
+        wait_flag(fpinv + p, tag, abort);
+        wait_flag(frow + p * nt + tj, tag, abort);
+        wait_flag(fcol + p * nt + ti, tag, abort);
+        fetch_tile<T>(ppiv, L.Pr, L.Pi);
+        fetch_tile<T>(J.xrow + ((size_t)p * nt + tj) * tsz, L.Ar, L.Ai);
+        fetch_tile<T>(J.xcol + ((size_t)p * nt + ti) * tsz, L.Br, L.Bi);
+        __syncthreads();
+        v4 rr = {0, 0, 0, 0}, ri = {0, 0, 0, 0};
+        cgemm_quad<T>(rr, ri, L.Pr, L.Pi, L.Ar, L.Ai, qi, qj, lane, T(1));
+        acc_to_lds<T>(rr, ri, L.Rr, L.Ri, qi, qj, lane);
+        __syncthreads();
+        cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Rr, L.Ri, qi, qj, lane, T(-1));
+      }
+      __syncthreads();  // LDS is reused by the next step
+    }
+    // G_k: read by the sweeps after this launch
+    T *Gk = J.G + (size_t)k * 2 * np2;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int a = ti * TB + qi * 16 + Mfma<T>::crow(lane, t), b = tj * TB + qj * 16 + (lane & 15);
+      Gk[(size_t)a * J.np + b] = cr[t];
+      Gk[np2 + (size_t)a * J.np + b] = ci[t];
+    }
+    // every workgroup is done with block k's exchange buffers
+    if (tid == 0) __hip_atomic_fetch_add(barrier, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    wait_flag(barrier, (unsigned)(nt * nt) * tag, abort);
+  }
+}
+
 // ---------------------------------------------------------------- sweeps
 // Forward: y_k = G_k (r_k - l_k y_{k-1}); y is [nrhs][nx][np] complex T.
 template <typename T>
@@ -768,7 +966,6 @@ __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const
 constexpr int SRW = 2;              // rows per wave
 constexpr int MAXCL = 768 / 64;     // columns per lane at the largest block
 constexpr int MAXRHS = 8;           // right-hand sides per pass
-constexpr unsigned SPIN_CAP = 1u << 21;
 
 struct SweepSync {
   unsigned cnt[2];   // forward, backward step counters
@@ -791,14 +988,6 @@ __device__ inline void sweep_wait(SweepSync *sy, int dir, unsigned target) {
   __syncthreads();
 }
 
-template <typename T>
-__device__ inline T ld_xcd(const T *p) {  // coherent across workgroups on any XCD
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ inline void st_xcd(T *p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 template <typename T>
 __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const double *__restrict__ op,
@@ -1034,7 +1223,8 @@ int check_grid(const fdfd_grid *g) {
 }
 int check_prec(int precision) {
   const int base = precision & 0xff, flags = precision & ~0xff;
-  if ((base != FDFD_PREC_FP64 && base != FDFD_PREC_C64) || (flags & ~FDFD_FACTOR_PIVOT))
+  if ((base != FDFD_PREC_FP64 && base != FDFD_PREC_C64) ||
+      (flags & ~(FDFD_FACTOR_PIVOT | FDFD_FACTOR_STEPWISE)))
     return fail(FDFD_ERR_ARG, "precision must be FDFD_PREC_FP64 or FDFD_PREC_C64 (optionally | "
                 "FDFD_FACTOR_PIVOT), got %d", precision);
   return FDFD_OK;
@@ -1044,17 +1234,26 @@ inline size_t elem_size(int precision) { return base_prec(precision) == FDFD_PRE
 
 struct FactorWork {
   void *W; void *Pinv; int *status;
+  void *xcol; void *pinv_all; unsigned *sync;  // persistent-kernel exchange (W doubles as xrow)
 };
+size_t persist_sync_words(int nt) { return 2 + (size_t)nt + 2 * (size_t)nt * nt; }
 FactorWork factor_work_layout(const fdfd_grid *g, int precision, void *work) {
-  const size_t es = elem_size(precision);
+  const size_t np = pad_n(g->ny), nt = np / TB, es = elem_size(precision);
   char *base = (char *)work;
   FactorWork fw;
   fw.status = (int *)base;
   size_t off = 256;
-  fw.Pinv = base + off;
-  off += align_up(2 * 2 * TB * TB * es, 256);
-  fw.W = base + off;
+  fw.Pinv = base + off;     off += align_up(2 * 2 * TB * TB * es, 256);
+  fw.W = base + off;        off += align_up(2 * np * np * es, 256);
+  fw.xcol = base + off;     off += align_up(2 * np * np * es, 256);
+  fw.pinv_all = base + off; off += align_up(nt * 2 * TB * TB * es, 256);
+  fw.sync = (unsigned *)(base + off);
   return fw;
+}
+size_t factor_work_bytes(const fdfd_grid *g, int precision) {
+  const size_t np = pad_n(g->ny), nt = np / TB, es = elem_size(precision);
+  return 256 + align_up(2 * 2 * TB * TB * es, 256) + 2 * align_up(2 * np * np * es, 256) +
+         align_up(nt * 2 * TB * TB * es, 256) + align_up(4 * persist_sync_words((int)nt), 256);
 }
 
 struct SolveWork {
@@ -1078,6 +1277,37 @@ int launch_step(const FBatch<T> &b, hipStream_t s) {
   const int grid = b.wg_start[b.njobs];
   hipLaunchKernelGGL((k_gj_step<T, PIV>), dim3(grid), dim3(NT), 0, s, b);
   HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+// Persistent factorization when the whole tile grid fits on the device at
+// once; returns FDFD_ERR_HIP with *ran = false when it does not fit or when
+// a wait timed out (the caller then uses the launch-per-step path).
+template <typename T, bool PIV>
+int factor_persistent(const fdfd_grid *g, const double *op, void *factor, void *work, hipStream_t s,
+                      bool *ran) {
+  *ran = false;
+  const int n = g->ny, np = pad_n(n), nt = np / TB;
+  int dev = 0, cus = 0, per_cu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_gj_persist<T, PIV>, NT, 0));
+  // every workgroup must be resident at once; keep a quarter of the device
+  // for other streams' work
+  if ((long)nt * nt > (long)per_cu * cus * 3 / 4) return FDFD_OK;
+  FactorWork fw = factor_work_layout(g, sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64, work);
+  HIP_TRY(hipMemsetAsync(fw.status, 0, sizeof(int), s));
+  HIP_TRY(hipMemsetAsync(fw.sync, 0, 4 * persist_sync_words(nt), s));
+  PJob<T> J;
+  J.op = op; J.G = (T *)factor; J.xrow = (T *)fw.W; J.xcol = (T *)fw.xcol; J.pinv = (T *)fw.pinv_all;
+  J.sync = fw.sync; J.status = fw.status;
+  J.m = g->nx; J.n = n; J.np = np; J.nt = nt;
+  hipLaunchKernelGGL((k_gj_persist<T, PIV>), dim3(nt * nt), dim3(NT), 0, s, J);
+  HIP_TRY(hipGetLastError());
+  unsigned ab = 0;
+  HIP_TRY(hipMemcpyAsync(&ab, fw.sync, sizeof ab, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *ran = ab == 0;
   return FDFD_OK;
 }
 
@@ -1304,8 +1534,7 @@ size_t fdfd_factor_bytes(const fdfd_grid *g, int precision) {
 }
 size_t fdfd_factor_workspace_bytes(const fdfd_grid *g, int precision) {
   if (check_grid(g) || check_prec(precision)) return 0;
-  const size_t np = pad_n(g->ny), es = elem_size(precision);
-  return 256 + align_up(2 * 2 * TB * TB * es, 256) + align_up(2 * np * np * es, 256);
+  return factor_work_bytes(g, precision);
 }
 size_t fdfd_solve_workspace_bytes(const fdfd_grid *g, int precision, int nrhs) {
   if (check_grid(g) || check_prec(precision) || nrhs < 1) return 0;
@@ -1349,7 +1578,17 @@ int fdfd_factor(const fdfd_grid *g, const double *op, void *factor, int precisio
   if (int rc = check_prec(precision)) return rc;
   if (!op || !factor || !work) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_factor");
   const bool piv = precision & FDFD_FACTOR_PIVOT;
-  if (base_prec(precision) == FDFD_PREC_FP64)
+  const bool fp64 = base_prec(precision) == FDFD_PREC_FP64;
+  if (!(precision & FDFD_FACTOR_STEPWISE)) {
+    bool ran = false;
+    int rc = fp64 ? (piv ? factor_persistent<double, true>(g, op, factor, work, stream, &ran)
+                         : factor_persistent<double, false>(g, op, factor, work, stream, &ran))
+                  : (piv ? factor_persistent<float, true>(g, op, factor, work, stream, &ran)
+                         : factor_persistent<float, false>(g, op, factor, work, stream, &ran));
+    if (rc) return rc;
+    if (ran) return FDFD_OK;
+  }
+  if (fp64)
     return piv ? factor_impl<double, true>(g, op, factor, work, stream)
                : factor_impl<double, false>(g, op, factor, work, stream);
   return piv ? factor_impl<float, true>(g, op, factor, work, stream)

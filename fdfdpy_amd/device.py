@@ -192,18 +192,19 @@ class DeviceOperator:
             self.version = self.linear_version
 
     # -- factor --------------------------------------------------------------
-    def factor(self, precision="fp64", refresh=False, pivot=False):
+    def factor(self, precision="fp64", refresh=False, pivot=False, stepwise=False):
         """Resident factor for this precision; built for the current operator
         when absent, or when refresh is set and the resident one belongs to
         an older operator or another pivoting mode. pivot=True pivots inside
         each 32x32 pivot tile (the fallback when the diagonal-pivot factor
-        does not refine)."""
+        does not refine). stepwise=True forces one launch per Gauss-Jordan
+        step instead of the persistent launch."""
         if precision in self.factors:
             fac, ver, piv = self.factors[precision]
             if not refresh or (ver == self.version and piv == pivot):
                 return fac
         g = ctypes.byref(self.grid)
-        p = L.PREC[precision] | (L.FACTOR_PIVOT if pivot else 0)
+        p = L.PREC[precision] | (L.FACTOR_PIVOT if pivot else 0) | (L.FACTOR_STEPWISE if stepwise else 0)
         fac = torch.empty(self.lib.fdfd_factor_bytes(g, L.PREC[precision]), dtype=torch.uint8,
                           device=self.device)
         work = torch.empty(self.lib.fdfd_factor_workspace_bytes(g, L.PREC[precision]), dtype=torch.uint8,

@@ -52,6 +52,10 @@ enum { FDFD_PREC_FP64 = 0, FDFD_PREC_C64 = 1 };      /* factor precision  */
  * partial pivoting inside each 32x32 pivot tile (slower; the fallback when
  * the default diagonal-pivot factor does not refine to the tolerance). */
 #define FDFD_FACTOR_PIVOT 0x100
+/* OR'ed into the precision argument of fdfd_factor: one launch per
+ * Gauss-Jordan step instead of the single persistent launch (which
+ * fdfd_factor uses, synchronously, when the tile grid fits the device). */
+#define FDFD_FACTOR_STEPWISE 0x200
 enum { FDFD_OP_C = 0, FDFD_OP_XM = 1, FDFD_OP_XP = 2, FDFD_OP_YM = 3, FDFD_OP_YP = 4 };
 
 /* Grid and physics parameters; the fields of Simulation.__init__
@@ -102,8 +106,10 @@ int fdfd_residual(const fdfd_grid *g, const double *op, const double *b,
 
 /* Replaces pardisoSolver(A, mtype=13).factor() (linalg.py:135-136): block
  * sweep factorization of the x-non-periodic part of A. factor holds
- * fdfd_factor_bytes, work fdfd_factor_workspace_bytes. Enqueued on stream;
- * status is checked by fdfd_factor_status after the stream completes. */
+ * fdfd_factor_bytes, work fdfd_factor_workspace_bytes. Uses one persistent
+ * launch (and waits for it) when the tile grid fits the device, otherwise
+ * enqueues one launch per Gauss-Jordan step; the numeric status is read by
+ * fdfd_factor_status. */
 int fdfd_factor(const fdfd_grid *g, const double *op, void *factor, int precision,
                 void *work, hipStream_t stream);
 

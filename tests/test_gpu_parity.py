@@ -202,3 +202,17 @@ def test_pivoted_tile_factor_matches_oracle(precision):
     assert rep.converged and not rep.fell_back
     xref = solver_direct(sim.A, b.reshape(-1)).reshape(N)
     assert relerr(x.cpu().numpy(), xref) < 1e-10
+
+
+@pytest.mark.parametrize("precision,n", [("c64", 200), ("fp64", 96), ("c64", 64)])
+def test_persistent_and_stepwise_factors_identical(precision, n):
+    """The single-launch factorization equals the launch-per-step one bit for bit."""
+    p = C.c3_request(77, n=n)
+    ops = []
+    for stepwise in (False, True):
+        op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, DEV)
+        op.set_eps(p.eps)
+        op.factor(precision, stepwise=stepwise)
+        ops.append(op)
+    torch.cuda.synchronize()
+    assert torch.equal(ops[0].factors[precision][0], ops[1].factors[precision][0])
