@@ -243,6 +243,12 @@ template <typename T>
 __device__ inline void st_xcd(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Every wave waits for its own outstanding memory operations (the exchange
+// stores among them); followed by a barrier, the data of the whole group is
+// then visible before thread 0 raises a flag. (The barrier alone only waits
+// for LDS traffic.)
+__device__ inline void stores_complete() { __builtin_amdgcn_s_waitcnt(0); }
+
 // thread 0 waits until *flag >= target (or abort); then the whole group
 __device__ inline void wait_flag(const unsigned *flag, unsigned target, unsigned *abort) {
   if (threadIdx.x == 0) {
@@ -772,11 +778,13 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
           ci[t] = L.Ai[a][b];
         }
         publish_tile<T>(ppiv, cr, ci, qi, qj, lane);
+        stores_complete();
         __syncthreads();
         if (tid == 0) __hip_atomic_store(fpinv + p, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else if (ti == p) {  // X[p, J] <- P X[p, J]
         publish_tile<T>(prow, cr, ci, qi, qj, lane);
         acc_to_lds<T>(cr, ci, L.Ar, L.Ai, qi, qj, lane);
+        stores_complete();
         __syncthreads();
         if (tid == 0) __hip_atomic_store(frow + p * nt + tj, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         wait_flag(fpinv + p, tag, abort);
@@ -788,6 +796,7 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
       } else if (tj == p) {  // X[I, p] <- -X[I, p] P
         publish_tile<T>(pcol, cr, ci, qi, qj, lane);
         acc_to_lds<T>(cr, ci, L.Br, L.Bi, qi, qj, lane);
+        stores_complete();
         __syncthreads();
         if (tid == 0) __hip_atomic_store(fcol + p * nt + ti, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         wait_flag(fpinv + p, tag, abort);
@@ -821,6 +830,8 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
       Gk[np2 + (size_t)a * J.np + b] = ci[t];
     }
     // every workgroup is done with block k's exchange buffers
+    stores_complete();
+    __syncthreads();
     if (tid == 0) __hip_atomic_fetch_add(barrier, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     wait_flag(barrier, (unsigned)(nt * nt) * tag, abort);
   }
@@ -1085,8 +1096,9 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
         }
       }
     }
-    // 5. publish this block's rows: the barrier waits for every wave's
-    // stores (workgroup-scope release), then one relaxed agent increment
+    // 5. publish this block's rows: every wave's stores complete, then one
+    // relaxed agent-scope increment
+    stores_complete();
     __syncthreads();
     if (tid == 0) __hip_atomic_fetch_add(&sy->cnt[dir], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
