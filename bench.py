@@ -49,40 +49,66 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--roofline-only", action="store_true", help="only the dominant-kernel measurement")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only a plain 256 MiB device copy (scale for the PMC byte counters)")
     return ap.parse_args()
 
 
-def roofline_leg(n=512, precision="c64"):
-    """Factor one n x n operator; the factor is k_gj_step launches back to back."""
+def _timed_factor(op, precision, stepwise):
     import torch
-    from fdfdpy_amd import configs as C
-    from fdfdpy_amd.device import DeviceOperator
-    p = C.c3_request(7, n=n)
-    op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, "cuda:0")
-    op.set_eps(p.eps)
-    op.factor(precision)            # warm
+    op.factor(precision, stepwise=stepwise)          # warm
     op.drop_factor()
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     e0.record(s)
-    op.factor(precision)
+    op.factor(precision, stepwise=stepwise)
     e1.record(s)
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
+    return e0.elapsed_time(e1)
+
+
+def roofline_leg(n=512, precision="c64"):
+    """Factor one n x n operator with the stepwise kernel the request loop uses.
+
+    The factorization is n*(n/32) back-to-back k_gj_step launches (+1 for
+    the first pivot), timed with HIP events on their stream. The single
+    persistent launch (k_gj_persist, the single-request path) is timed too.
+    """
+    from fdfdpy_amd import configs as C
+    from fdfdpy_amd.device import DeviceOperator
+    p = C.c3_request(7, n=n)
+    op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, "cuda:0")
+    op.set_eps(p.eps)
+    ms = _timed_factor(op, precision, stepwise=True)
+    ms_persist = _timed_factor(op, precision, stepwise=False)
     nt = (n + 31) // 32
     launches = n * nt + 1
     flops = 8.0 * n ** 4                       # 8 m n^3 real flops, m = n blocks
     achieved = flops / (ms * 1e-3) / 1e12
     peak = FP32_MFMA_PEAK_TFLOPS if precision == "c64" else FP64_MFMA_PEAK_TFLOPS
+    tname = "float" if precision == "c64" else "double"
     return {
-        "kernel": f"k_gj_step<{'float' if precision == 'c64' else 'double'}>",
+        "kernel": f"k_gj_step<{tname}, false>",
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
         "frac": round(achieved / peak, 5), "traffic": None,
         "workload": f"complex factor of a {n}x{n} Ez operator: {launches} launches, "
-                    f"8*n^4 = {flops:.3e} flops",
+                    f"8*n^4 = {flops:.3e} flops per factor, 8*n^2*32 = {8.0 * n * n * 32:.3e} per launch",
         "avg_launch_us": round(ms * 1e3 / launches, 3), "factor_ms": round(ms, 3),
+        "persistent_factor_ms": round(ms_persist, 3),
+        "persistent_achieved_tflops": round(flops / (ms_persist * 1e-3) / 1e12, 3),
     }
+
+
+def copy_calibration(nbytes=256 << 20, reps=3):
+    """A plain device copy of known size (scales the PMC byte counters)."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"copy_bytes_each_way": nbytes, "copies": reps}
 
 
 def cpu_baseline(reqs, seconds):
@@ -135,6 +161,9 @@ def main():
     if args.roofline_only:
         print(json.dumps(roofline_leg()))
         return
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration()))
+        return
 
     n_req = args.steps * args.batch
     log(f"stream: {n_req} requests, Poisson rate {args.rate}/s, {args.slots} slots")
@@ -183,7 +212,8 @@ def main():
         "max_rel_residual": mixed["max_rel_residual"], "fallbacks": mixed["fallbacks"],
         "failed": mixed["failed"], "mean_sweeps": mixed["mean_sweeps"],
         "roofline": {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
-        "roofline_detail": {k: roof[k] for k in ("kernel", "workload", "avg_launch_us", "factor_ms")},
+        "roofline_detail": {k: roof[k] for k in ("kernel", "workload", "avg_launch_us", "factor_ms",
+                                                 "persistent_factor_ms", "persistent_achieved_tflops")},
         "cpu_baseline": cpu,
     }
     if fp64:

@@ -76,6 +76,7 @@ SIGNATURES = {
     "fdfd_engine_submit": (c_int, [c_void_p, _G, c_void_p, c_void_p, c_int, c_void_p, POINTER(ctypes.c_uint64)]),
     "fdfd_engine_wait": (c_int, [c_void_p, ctypes.c_uint64]),
     "fdfd_engine_launches": (ctypes.c_uint64, [c_void_p]),
+    "fdfd_set_spin_cap": (None, [ctypes.c_uint]),
 }
 
 _lib = None

@@ -234,6 +234,7 @@ __global__ void k_selftest_mfma_f32(const double *a, const double *b, double *d)
 // Every wait is bounded: after SPIN_CAP polls it raises an abort flag, and
 // every later wait returns at once, so a grid always drains.
 constexpr unsigned SPIN_CAP = 1u << 21;
+unsigned g_spin_cap = SPIN_CAP;  // host setting passed to the persistent kernels
 
 template <typename T>
 __device__ inline T ld_xcd(const T *p) {
@@ -250,12 +251,12 @@ __device__ inline void st_xcd(T *p, T v) {
 __device__ inline void stores_complete() { __builtin_amdgcn_s_waitcnt(0); }
 
 // thread 0 waits until *flag >= target (or abort); then the whole group
-__device__ inline void wait_flag(const unsigned *flag, unsigned target, unsigned *abort) {
+__device__ inline void wait_flag(const unsigned *flag, unsigned target, unsigned *abort, unsigned cap) {
   if (threadIdx.x == 0) {
     unsigned spins = 0;
     while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-      if (++spins > SPIN_CAP) {
+      if (++spins > cap) {
         __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
@@ -686,6 +687,7 @@ template <typename T> struct PJob {
   unsigned *sync;  // [0] abort, [1] barrier, [2, 2+nt) pinv flags, then row / col flags
   int *status;
   int m, n, np, nt;
+  unsigned cap;    // polls before a wait gives up
 };
 
 template <typename T>
@@ -787,7 +789,7 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
         stores_complete();
         __syncthreads();
         if (tid == 0) __hip_atomic_store(frow + p * nt + tj, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        wait_flag(fpinv + p, tag, abort);
+        wait_flag(fpinv + p, tag, abort, J.cap);
         fetch_tile<T>(ppiv, L.Pr, L.Pi);
         __syncthreads();
         cr = v4{0, 0, 0, 0};
@@ -799,16 +801,16 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
         stores_complete();
         __syncthreads();
         if (tid == 0) __hip_atomic_store(fcol + p * nt + ti, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        wait_flag(fpinv + p, tag, abort);
+        wait_flag(fpinv + p, tag, abort, J.cap);
         fetch_tile<T>(ppiv, L.Pr, L.Pi);
         __syncthreads();
         cr = v4{0, 0, 0, 0};
         ci = v4{0, 0, 0, 0};
         cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Pr, L.Pi, qi, qj, lane, T(-1));
       } else {  // X[I, J] -= X[I, p] (P X[p, J])
-        wait_flag(fpinv + p, tag, abort);
-        wait_flag(frow + p * nt + tj, tag, abort);
-        wait_flag(fcol + p * nt + ti, tag, abort);
+        wait_flag(fpinv + p, tag, abort, J.cap);
+        wait_flag(frow + p * nt + tj, tag, abort, J.cap);
+        wait_flag(fcol + p * nt + ti, tag, abort, J.cap);
         fetch_tile<T>(ppiv, L.Pr, L.Pi);
         fetch_tile<T>(J.xrow + ((size_t)p * nt + tj) * tsz, L.Ar, L.Ai);
         fetch_tile<T>(J.xcol + ((size_t)p * nt + ti) * tsz, L.Br, L.Bi);
@@ -833,7 +835,7 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
     stores_complete();
     __syncthreads();
     if (tid == 0) __hip_atomic_fetch_add(barrier, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    wait_flag(barrier, (unsigned)(nt * nt) * tag, abort);
+    wait_flag(barrier, (unsigned)(nt * nt) * tag, abort, J.cap);
   }
 }
 
@@ -984,12 +986,12 @@ struct SweepSync {
   unsigned pad;
 };
 
-__device__ inline void sweep_wait(SweepSync *sy, int dir, unsigned target) {
+__device__ inline void sweep_wait(SweepSync *sy, int dir, unsigned target, unsigned cap) {
   if (threadIdx.x == 0) {
     unsigned spins = 0;
     while (__hip_atomic_load(&sy->cnt[dir], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (__hip_atomic_load(&sy->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-      if (++spins > SPIN_CAP) {
+      if (++spins > cap) {
         __hip_atomic_store(&sy->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
@@ -1003,7 +1005,8 @@ __device__ inline void sweep_wait(SweepSync *sy, int dir, unsigned target) {
 template <typename T>
 __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const double *__restrict__ op,
                                                const double *__restrict__ r, T *y, double *__restrict__ x,
-                                               int nx, int ny, int np, int nrhs, int dir, SweepSync *sy) {
+                                               int nx, int ny, int np, int nrhs, int dir, SweepSync *sy,
+                                               unsigned cap) {
   extern __shared__ unsigned char smem_raw[];
   T *vr = reinterpret_cast<T *>(smem_raw);   // [nrhs][np]
   T *vi = vr + (size_t)nrhs * np;
@@ -1040,7 +1043,7 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
       vi[e] = pi;
     }
     // 3. wait for the previous block, then fold it into v
-    if (s > 0) sweep_wait(sy, dir, (unsigned)(W * s));
+    if (s > 0) sweep_wait(sy, dir, (unsigned)(W * s), cap);
     else __syncthreads();
     if (coupled) {
       const int kp = dir == 0 ? k - 1 : k + 1;
@@ -1313,7 +1316,7 @@ int factor_persistent(const fdfd_grid *g, const double *op, void *factor, void *
   PJob<T> J;
   J.op = op; J.G = (T *)factor; J.xrow = (T *)fw.W; J.xcol = (T *)fw.xcol; J.pinv = (T *)fw.pinv_all;
   J.sync = fw.sync; J.status = fw.status;
-  J.m = g->nx; J.n = n; J.np = np; J.nt = nt;
+  J.m = g->nx; J.n = n; J.np = np; J.nt = nt; J.cap = g_spin_cap;
   hipLaunchKernelGGL((k_gj_persist<T, PIV>), dim3(nt * nt), dim3(NT), 0, s, J);
   HIP_TRY(hipGetLastError());
   unsigned ab = 0;
@@ -1383,9 +1386,9 @@ int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const d
     const size_t lds = (size_t)2 * nr * np * sizeof(T);
     HIP_TRY(hipMemsetAsync(sw.sync, 0, sizeof(SweepSync), s));
     hipLaunchKernelGGL(k_psweep<T>, dim3(W), dim3(NT), lds, s, (const T *)factor, op, r + 2 * M * q0, yq,
-                       (double *)nullptr, nx, ny, np, nr, 0, sw.sync);
+                       (double *)nullptr, nx, ny, np, nr, 0, sw.sync, g_spin_cap);
     hipLaunchKernelGGL(k_psweep<T>, dim3(W), dim3(NT), lds, s, (const T *)factor, op, (const double *)nullptr,
-                       yq, x + 2 * M * q0, nx, ny, np, nr, 1, sw.sync);
+                       yq, x + 2 * M * q0, nx, ny, np, nr, 1, sw.sync, g_spin_cap);
   }
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
@@ -1851,6 +1854,8 @@ uint64_t fdfd_engine_launches(fdfd_engine *E) {
   std::lock_guard<std::mutex> lk(E->mu);
   return E->launches;
 }
+
+void fdfd_set_spin_cap(unsigned polls) { g_spin_cap = polls ? polls : SPIN_CAP; }
 
 int fdfd_selftest_mfma(int precision, const double *a, const double *b, double *d, hipStream_t stream) {
   if (precision < 0 || precision > 2) return fail(FDFD_ERR_ARG, "selftest mode must be 0, 1 or 2");

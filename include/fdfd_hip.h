@@ -179,6 +179,11 @@ int fdfd_engine_submit(fdfd_engine *engine, const fdfd_grid *g, const double *op
 int fdfd_engine_wait(fdfd_engine *engine, uint64_t ticket);
 uint64_t fdfd_engine_launches(fdfd_engine *engine);
 
+/* Polls a bounded wait of the persistent kernels makes before it gives up
+ * (0 restores the default, 2^21). With a tiny value every wait gives up
+ * at once, which exercises the fallback paths (no reference counterpart). */
+void fdfd_set_spin_cap(unsigned polls);
+
 /* Check of the MFMA operand layouts the factor kernels rely on (no reference
  * counterpart): D = A (16x4) * B (4x16) with the 16x16x4 f64 (precision 0)
  * or f32 (precision 1) instruction; a, b, d are row-major device arrays of

@@ -216,3 +216,24 @@ def test_persistent_and_stepwise_factors_identical(precision, n):
         ops.append(op)
     torch.cuda.synchronize()
     assert torch.equal(ops[0].factors[precision][0], ops[1].factors[precision][0])
+
+
+def test_bounded_waits_fall_back_when_forced():
+    """With a one-poll spin limit every cross-workgroup wait gives up at once:
+    the persistent factor and sweeps must fall back and still be right."""
+    lib = L.lib()
+    rng = np.random.default_rng(12)
+    N = (64, 96)
+    eps = 1 + 11.25 * (rng.random(N) < 0.5)
+    sim = OracleSimulation(OMEGA, eps, 0.02, [10, 10], 'Ez')
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    xref = solver_direct(sim.A, b.reshape(-1)).reshape(N)
+    lib.fdfd_set_spin_cap(1)
+    try:
+        op = DeviceOperator(N, OMEGA, 0.02, [10, 10], 'Ez', 1e-6, DEV)
+        op.set_eps(eps)
+        x, rep = op.solve(torch.tensor(b, device=DEV), precision="c64", tol=1e-12)
+    finally:
+        lib.fdfd_set_spin_cap(0)
+    assert rep.converged
+    assert relerr(x.cpu().numpy(), xref) < 1e-10
