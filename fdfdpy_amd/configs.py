@@ -77,6 +77,12 @@ def c1_point():
 
 def point_sources(rng, n, npml, lo, hi, k):
     """k distinct cells outside [lo, hi)^2 and outside the PML."""
+    a, b = npml + 2, n - npml - 2
+    side = max(0, b - a)
+    inner = max(0, min(b, hi) - max(a, lo))
+    if side * side - inner * inner < k:
+        raise ValueError(f"no room for {k} sources outside the design region of a {n}-cell grid "
+                         f"with NPML {npml}")
     cells = set()
     while len(cells) < k:
         i, j = rng.integers(npml + 2, n - npml - 2, size=2)

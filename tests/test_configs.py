@@ -47,3 +47,11 @@ def test_c4_c5():
     q = C.c5_kerr()
     assert q.shape == (400, 400) and len(q.extra['amps']) == 8
     assert np.all((q.extra['amps'] >= 3) & (q.extra['amps'] <= 60))
+
+
+def test_point_sources_refuse_impossible_layouts():
+    import pytest
+    rng = np.random.default_rng(0)
+    with pytest.raises(ValueError):
+        C.point_sources(rng, 64, 14, 16, 48, 1)     # every PML-free cell is in the design region
+    assert len(C.point_sources(rng, 128, 15, 32, 96, 4)) == 4
