@@ -250,13 +250,36 @@ __device__ inline void st_xcd(T *p, T v) {
 // for LDS traffic.)
 __device__ inline void stores_complete() { __builtin_amdgcn_s_waitcnt(0); }
 
+// Wave 0 waits until flags[0..nflags) are all >= target (or abort); then
+// the whole group. Per-workgroup flags are plain stores: no same-address
+// atomics to serialize at the coherence point.
+__device__ inline void wait_all_flags(const unsigned *flags, int nflags, unsigned target, unsigned *abort,
+                                      unsigned cap) {
+  if (threadIdx.x < 64) {
+    unsigned spins = 0;
+    for (;;) {
+      int ok = 1;
+      for (int w = threadIdx.x; w < nflags; w += 64)
+        ok &= __hip_atomic_load(flags + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+      if (__all(ok)) break;
+      if ((++spins & 63) == 0 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (spins > cap) {
+        if (threadIdx.x == 0) __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+}
+
 // thread 0 waits until *flag >= target (or abort); then the whole group
 __device__ inline void wait_flag(const unsigned *flag, unsigned target, unsigned *abort, unsigned cap) {
   if (threadIdx.x == 0) {
     unsigned spins = 0;
     while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-      if (++spins > cap) {
+      if ((++spins & 63) == 0 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (spins > cap) {
         __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
@@ -675,16 +698,19 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
 // before updating them, the owner of (p, p) inverts its tile and publishes
 // the inverse, and every other workgroup waits for the three tiles it needs.
 // Flags hold the block number + 1, so they never need resetting within a
-// launch; one grid barrier per block guards the reuse of the exchange
-// buffers. At the end of block k every workgroup writes its tile of G_k and
-// applies the Schur update of block k+1 to its registers.
+// launch. The exchange buffers alternate between two sets by block parity:
+// a workgroup writing set (k & 1) at block k+2 has (through the flags it
+// waited on in block k+1) waited for every reader of block k to finish
+// block k, so no grid barrier is needed. At the end of block k every
+// workgroup writes its tile of G_k and applies the Schur update of block
+// k+1 to its registers.
 template <typename T> struct PJob {
   const double *op;
   T *G;            // [m][2][np][np]
-  T *xrow;         // [nt][nt][2][TB*TB]  published X[p, J]
-  T *xcol;         // [nt][nt][2][TB*TB]  published X[I, p]
-  T *pinv;         // [nt][2][TB*TB]      published inverses
-  unsigned *sync;  // [0] abort, [1] barrier, [2, 2+nt) pinv flags, then row / col flags
+  T *xrow;         // [2 parity][nt][nt][2][TB*TB]  published X[p, J]
+  T *xcol;         // [2 parity][nt][nt][2][TB*TB]  published X[I, p]
+  T *pinv;         // [2 parity][nt][2][TB*TB]      published inverses
+  unsigned *sync;  // [0] abort, [1] unused, [2, 2+nt) pinv flags, then row / col flags
   int *status;
   int m, n, np, nt;
   unsigned cap;    // polls before a wait gives up
@@ -726,7 +752,7 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int qi = w >> 1, qj = w & 1;
   const size_t np2 = (size_t)J.np * J.np, tsz = 2 * TB * TB;
-  unsigned *abort = J.sync, *barrier = J.sync + 1, *fpinv = J.sync + 2;
+  unsigned *abort = J.sync, *fpinv = J.sync + 2;
   unsigned *frow = fpinv + nt, *fcol = frow + nt * nt;
   typedef typename Mfma<T>::v4 v4;
   v4 cr, ci;
@@ -765,10 +791,13 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
         ci[t] = (T)vi;
       }
     }
+    const size_t par = k & 1;
+    T *xrow = J.xrow + par * nt * nt * tsz, *xcol = J.xcol + par * nt * nt * tsz;
+    T *pinv = J.pinv + par * nt * tsz;
     for (int p = 0; p < nt; ++p) {
-      T *prow = J.xrow + ((size_t)p * nt + tj) * tsz;
-      T *pcol = J.xcol + ((size_t)p * nt + ti) * tsz;
-      T *ppiv = J.pinv + (size_t)p * tsz;
+      T *prow = xrow + ((size_t)p * nt + tj) * tsz;
+      T *pcol = xcol + ((size_t)p * nt + ti) * tsz;
+      T *ppiv = pinv + (size_t)p * tsz;
       if (ti == p && tj == p) {
         acc_to_lds<T>(cr, ci, L.Ar, L.Ai, qi, qj, lane);
         __syncthreads();
@@ -812,8 +841,8 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
         wait_flag(frow + p * nt + tj, tag, abort, J.cap);
         wait_flag(fcol + p * nt + ti, tag, abort, J.cap);
         fetch_tile<T>(ppiv, L.Pr, L.Pi);
-        fetch_tile<T>(J.xrow + ((size_t)p * nt + tj) * tsz, L.Ar, L.Ai);
-        fetch_tile<T>(J.xcol + ((size_t)p * nt + ti) * tsz, L.Br, L.Bi);
+        fetch_tile<T>(prow, L.Ar, L.Ai);
+        fetch_tile<T>(xcol + ((size_t)p * nt + ti) * tsz, L.Br, L.Bi);
         __syncthreads();
         v4 rr = {0, 0, 0, 0}, ri = {0, 0, 0, 0};
         cgemm_quad<T>(rr, ri, L.Pr, L.Pi, L.Ar, L.Ai, qi, qj, lane, T(1));
@@ -831,11 +860,6 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
       Gk[(size_t)a * J.np + b] = cr[t];
       Gk[np2 + (size_t)a * J.np + b] = ci[t];
     }
-    // every workgroup is done with block k's exchange buffers
-    stores_complete();
-    __syncthreads();
-    if (tid == 0) __hip_atomic_fetch_add(barrier, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    wait_flag(barrier, (unsigned)(nt * nt) * tag, abort, J.cap);
   }
 }
 
@@ -980,27 +1004,13 @@ constexpr int SRW = 2;              // rows per wave
 constexpr int MAXCL = 768 / 64;     // columns per lane at the largest block
 constexpr int MAXRHS = 8;           // right-hand sides per pass
 
+constexpr int MAXW = 768 / (4 * SRW);   // sweep workgroups at the largest block
+
 struct SweepSync {
-  unsigned cnt[2];   // forward, backward step counters
   unsigned abort;
-  unsigned pad;
+  unsigned pad[63];
+  unsigned done[2][MAXW];   // forward / backward: steps finished per workgroup
 };
-
-__device__ inline void sweep_wait(SweepSync *sy, int dir, unsigned target, unsigned cap) {
-  if (threadIdx.x == 0) {
-    unsigned spins = 0;
-    while (__hip_atomic_load(&sy->cnt[dir], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(&sy->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-      if (++spins > cap) {
-        __hip_atomic_store(&sy->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-  }
-  __syncthreads();
-}
-
 
 template <typename T>
 __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const double *__restrict__ op,
@@ -1043,7 +1053,7 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
       vi[e] = pi;
     }
     // 3. wait for the previous block, then fold it into v
-    if (s > 0) sweep_wait(sy, dir, (unsigned)(W * s), cap);
+    if (s > 0) wait_all_flags(sy->done[dir], W, (unsigned)s, &sy->abort, cap);
     else __syncthreads();
     if (coupled) {
       const int kp = dir == 0 ? k - 1 : k + 1;
@@ -1099,11 +1109,12 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
         }
       }
     }
-    // 5. publish this block's rows: every wave's stores complete, then one
-    // relaxed agent-scope increment
+    // 5. publish this block's rows: every wave's stores complete, then this
+    // workgroup's flag
     stores_complete();
     __syncthreads();
-    if (tid == 0) __hip_atomic_fetch_add(&sy->cnt[dir], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0)
+      __hip_atomic_store(&sy->done[dir][blockIdx.x], (unsigned)(s + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -1259,16 +1270,16 @@ FactorWork factor_work_layout(const fdfd_grid *g, int precision, void *work) {
   fw.status = (int *)base;
   size_t off = 256;
   fw.Pinv = base + off;     off += align_up(2 * 2 * TB * TB * es, 256);
-  fw.W = base + off;        off += align_up(2 * np * np * es, 256);
-  fw.xcol = base + off;     off += align_up(2 * np * np * es, 256);
-  fw.pinv_all = base + off; off += align_up(nt * 2 * TB * TB * es, 256);
+  fw.W = base + off;        off += align_up(2 * 2 * np * np * es, 256);  // also xrow (2 parities)
+  fw.xcol = base + off;     off += align_up(2 * 2 * np * np * es, 256);
+  fw.pinv_all = base + off; off += align_up(2 * nt * 2 * TB * TB * es, 256);
   fw.sync = (unsigned *)(base + off);
   return fw;
 }
 size_t factor_work_bytes(const fdfd_grid *g, int precision) {
   const size_t np = pad_n(g->ny), nt = np / TB, es = elem_size(precision);
-  return 256 + align_up(2 * 2 * TB * TB * es, 256) + 2 * align_up(2 * np * np * es, 256) +
-         align_up(nt * 2 * TB * TB * es, 256) + align_up(4 * persist_sync_words((int)nt), 256);
+  return 256 + align_up(2 * 2 * TB * TB * es, 256) + 2 * align_up(2 * 2 * np * np * es, 256) +
+         align_up(2 * nt * 2 * TB * TB * es, 256) + align_up(4 * persist_sync_words((int)nt), 256);
 }
 
 struct SolveWork {
