@@ -140,7 +140,11 @@ def run_stream(reqs, precision, slots, tol):
     eng = ServingEngine("cuda:0", precision=precision, slots=slots, tol=tol)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = eng.run(reqs)
+
+    def progress(done, total):
+        if done % 32 == 0 or done == total:
+            log(f"  {precision}: {done}/{total} requests done, {time.perf_counter() - t0:.1f} s")
+    res = eng.run(reqs, progress=progress)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return summarize(res, wall), res

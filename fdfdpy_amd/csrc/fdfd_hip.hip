@@ -991,18 +991,28 @@ __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const
 
 // Persistent sweeps: one launch per direction. Workgroup w owns rows
 // [w*4*SRW, (w+1)*4*SRW) of every block; lane l of a wave covers columns
-// l, l+64, ... of its rows. Blocks are chained through a counter: step s
-// waits until all W workgroups have finished step s-1. Each workgroup loads
-// its rows of G_k before it waits, so HBM latency overlaps the
+// l, l+64, ... of its rows. Blocks are chained through per-workgroup flags:
+// step s waits until all W workgroups have finished step s-1. Each workgroup
+// loads its rows of G_k, the coupling coefficients and (backward) its own
+// entries of y_k before it waits, so HBM latency overlaps the
 // synchronization. A wait that exceeds SPIN_CAP polls raises the abort flag;
 // every later wait then returns at once, so the grid always drains.
 // The vectors passed between workgroups (y_k, d_k) are written and read with
 // agent-scope relaxed atomics, which are coherent across the XCDs' L2 caches
-// by themselves; the counter needs only workgroup-scope ordering around it,
-// so no step writes back or invalidates a whole L2.
+// by themselves; the flags need only workgroup-scope ordering around them,
+// so no step writes back or invalidates a whole L2. The row sums of a wave
+// (SRW rows x RP right-hand sides x re/im) are reduced together. RP, the
+// right-hand sides of one pass rounded up to a power of two, is a template
+// parameter so the unrolled loops stay small enough for the instruction
+// cache; v rows are padded to a multiple of 64 columns with zeros, so the
+// inner loop needs no per-lane guard.
 constexpr int SRW = 2;              // rows per wave
 constexpr int MAXCL = 768 / 64;     // columns per lane at the largest block
-constexpr int MAXRHS = 8;           // right-hand sides per pass
+constexpr int MAXRHS = 8;           // right-hand sides per pass (complex64)
+// right-hand sides per pass: FP64 holds half as many row sums in registers
+template <typename T> constexpr int sweep_rhs() { return sizeof(T) == 8 ? MAXRHS / 2 : MAXRHS; }
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+inline int pad64(int n) { return (n + 63) / 64 * 64; }
 
 constexpr int MAXW = 768 / (4 * SRW);   // sweep workgroups at the largest block
 
@@ -1012,22 +1022,61 @@ struct SweepSync {
   unsigned done[2][MAXW];   // forward / backward: steps finished per workgroup
 };
 
-template <typename T>
+// Sums v[0..N) over the 64 lanes of a wave at once: each round halves the
+// values a lane holds and exchanges the other half with its partner, so N
+// sums cost N-1 + (6 - log2 N) shuffles instead of 6 N. On return every lane
+// holds the sum of value (lane >> (6 - log2 N)).
+template <int H, typename T, int N>
+__device__ inline void multi_sum_round(T (&v)[N], int lane) {
+  if constexpr (H >= 1) {
+    constexpr int off = 64 * H / N;   // 32 for the first round
+    const bool upper = (lane & off) != 0;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const T send = upper ? v[j] : v[j + H];
+      const T keep = upper ? v[j + H] : v[j];
+      v[j] = keep + __shfl_xor(send, off);
+    }
+    multi_sum_round<H / 2>(v, lane);
+  }
+}
+template <typename T, int N>
+__device__ inline T wave_multi_sum(T (&v)[N], int lane) {
+  static_assert(N >= 2 && N <= 64 && (N & (N - 1)) == 0, "N must be a power of two <= 64");
+  multi_sum_round<N / 2>(v, lane);
+  T t = v[0];
+#pragma unroll
+  for (int off = 32 / N; off >= 1; off >>= 1) t += __shfl_xor(t, off);
+  return t;
+}
+
+template <typename T, int RP>
 __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const double *__restrict__ op,
                                                const double *__restrict__ r, T *y, double *__restrict__ x,
                                                int nx, int ny, int np, int nrhs, int dir, SweepSync *sy,
                                                unsigned cap) {
   extern __shared__ unsigned char smem_raw[];
-  T *vr = reinterpret_cast<T *>(smem_raw);   // [nrhs][np]
-  T *vi = vr + (size_t)nrhs * np;
+  double *cpr = reinterpret_cast<double *>(smem_raw);   // [np] coupling to the previous block
+  double *cpi = cpr + np;
+  const int nv = (np + 63) / 64 * 64;                  // padded row length of v
+  T *vr = reinterpret_cast<T *>(cpi + np);             // [RP][nv]
+  T *vi = vr + (size_t)RP * nv;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const size_t np2 = (size_t)np * np, M = (size_t)nx * ny;
   const int W = gridDim.x;
   const int row0 = (blockIdx.x * 4 + wv) * SRW;
+  // the (row, rhs, re/im) value this lane finishes after the reduction
+  static_assert(RP >= 1 && RP <= MAXRHS && (RP & (RP - 1)) == 0, "RP must be a power of two");
+  constexpr int NV = 2 * SRW * RP, SH = 6 - ilog2(NV);
+  const int jv = lane >> SH;   // lanes jv << SH ... all hold value jv
+  const int my_ri = jv & 1, my_q = (jv >> 1) % RP, my_rr = jv / (2 * RP);
+  const int my_a = row0 + my_rr;
+  const bool my_store = (lane & ((1 << SH) - 1)) == 0 && my_q < nrhs && my_a < np;
   for (int s = 0; s < nx; ++s) {
     const int k = dir == 0 ? s : nx - 1 - s;
     const T *Gk = G + (size_t)k * 2 * np2;
     const bool coupled = dir == 0 ? k > 0 : k + 1 < nx;
+    const bool active = dir == 0 || coupled;
     // 1. this workgroup's rows of G_k (independent of the previous block)
     T gr[SRW][MAXCL], gi[SRW][MAXCL];
 #pragma unroll
@@ -1036,16 +1085,28 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
 #pragma unroll
       for (int cc = 0; cc < MAXCL; ++cc) {
         const int c = lane + 64 * cc;
-        const bool ok = a < np && c < np && (dir == 0 || coupled);
+        const bool ok = a < np && c < np && active;
         gr[rr][cc] = ok ? Gk[(size_t)a * np + c] : T(0);
         gi[rr][cc] = ok ? Gk[np2 + (size_t)a * np + c] : T(0);
       }
     }
-    // 2. the part of v that does not depend on the previous block
-    for (int e = tid; e < nrhs * np; e += NT) {
-      const int q = e / np, c = e % np;
+    // backward: this lane's own entry of y_k, written by the forward launch
+    T yown = 0;
+    T *my_y = y + 2 * (((size_t)my_q * nx + k) * np + my_a) + my_ri;
+    if (dir == 1 && my_store) yown = *my_y;
+    // 2. the parts of v that do not depend on the previous block
+    if (coupled) {
+      const int pl = dir == 0 ? FDFD_OP_XM : FDFD_OP_XP;
+      for (int c = tid; c < ny; c += NT) {
+        const cd cpl = ld_c(op, pl * M + (size_t)k * ny + c);
+        cpr[c] = cpl.r;
+        cpi[c] = cpl.i;
+      }
+    }
+    for (int e = tid; e < RP * nv; e += NT) {
+      const int q = e / nv, c = e - q * nv;
       T pr = 0, pi = 0;
-      if (dir == 0 && c < ny) {
+      if (dir == 0 && q < nrhs && c < ny) {
         cd v = ld_c(r, (size_t)q * M + (size_t)k * ny + c);
         pr = (T)v.r; pi = (T)v.i;
       }
@@ -1057,56 +1118,48 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
     else __syncthreads();
     if (coupled) {
       const int kp = dir == 0 ? k - 1 : k + 1;
-      const int pl = dir == 0 ? FDFD_OP_XM : FDFD_OP_XP;
-      for (int e = tid; e < nrhs * np; e += NT) {
-        const int q = e / np, c = e % np;
+      for (int e = tid; e < nrhs * nv; e += NT) {
+        const int q = e / nv, c = e - q * nv;
         if (c < ny) {
           const T *yp = y + 2 * (((size_t)q * nx + kp) * np + c);
-          const cd cpl = ld_c(op, pl * M + (size_t)k * ny + c);
-          const cd t = cmul(cpl, cd{(double)ld_xcd(yp), (double)ld_xcd(yp + 1)});
-          if (dir == 0) { vr[e] = (T)((double)vr[e] - t.r); vi[e] = (T)((double)vi[e] - t.i); }
-          else { vr[e] = (T)t.r; vi[e] = (T)t.i; }
+          const double yr = (double)ld_xcd(yp), yi = (double)ld_xcd(yp + 1);
+          const double tr = cpr[c] * yr - cpi[c] * yi, ti = cpr[c] * yi + cpi[c] * yr;
+          if (dir == 0) { vr[e] = (T)((double)vr[e] - tr); vi[e] = (T)((double)vi[e] - ti); }
+          else { vr[e] = (T)tr; vi[e] = (T)ti; }
         }
       }
     }
     __syncthreads();
-    // 4. row dot products on this workgroup's rows
+    // 4. partial row dot products for every (row, rhs) pair of this wave
+    T acc[NV];
 #pragma unroll
-    for (int rr = 0; rr < SRW; ++rr) {
-      const int a = row0 + rr;
-      for (int q = 0; q < nrhs; ++q) {
-        T sr = 0, si = 0;
-        if (dir == 0 || coupled) {
+    for (int j = 0; j < NV; ++j) acc[j] = 0;
+    if (active) {
 #pragma unroll
-          for (int cc = 0; cc < MAXCL; ++cc) {
-            const int c = lane + 64 * cc;
-            if (c < np) {
-              const T br = vr[(size_t)q * np + c], bi = vi[(size_t)q * np + c];
+      for (int cc = 0; cc < MAXCL; ++cc) {
+        if (64 * cc < np) {   // uniform; columns c >= np hold G = 0 and v = 0
+          const int c = lane + 64 * cc;
+#pragma unroll
+          for (int q = 0; q < RP; ++q) {
+            const T br = vr[q * nv + c], bi = vi[q * nv + c];
+#pragma unroll
+            for (int rr = 0; rr < SRW; ++rr) {
+              T &sr = acc[(rr * RP + q) * 2], &si = acc[(rr * RP + q) * 2 + 1];
               sr += gr[rr][cc] * br - gi[rr][cc] * bi;
               si += gr[rr][cc] * bi + gi[rr][cc] * br;
             }
           }
-          for (int off = 32; off >= 1; off >>= 1) {
-            sr += __shfl_xor(sr, off);
-            si += __shfl_xor(si, off);
-          }
         }
-        if (lane == 0 && a < np) {
-          T *yk = y + 2 * (((size_t)q * nx + k) * np + a);
-          if (dir == 0) {
-            st_xcd(yk, sr);
-            st_xcd(yk + 1, si);
-          } else {
-            const T dr = yk[0] - sr, di = yk[1] - si;  // own rows, written by the forward launch
-            st_xcd(yk, dr);
-            st_xcd(yk + 1, di);
-            if (a < ny) {
-              double *xo = x + 2 * ((size_t)q * M + (size_t)k * ny + a);
-              xo[0] += (double)dr;
-              xo[1] += (double)di;
-            }
-          }
-        }
+      }
+    }
+    const T sum = wave_multi_sum(acc, lane);
+    if (my_store) {
+      if (dir == 0) {
+        st_xcd(my_y, sum);
+      } else {
+        const T d = yown - sum;
+        st_xcd(my_y, d);
+        if (my_a < ny) x[2 * ((size_t)my_q * M + (size_t)k * ny + my_a) + my_ri] += (double)d;
       }
     }
     // 5. publish this block's rows: every wave's stores complete, then this
@@ -1377,8 +1430,20 @@ int sweep_blockwise(const fdfd_grid *g, const double *op, const void *factor, co
   return FDFD_OK;
 }
 
-// x += F^-1 r with the persistent sweep kernels (RHS in passes of MAXRHS);
-// blockwise = true uses one launch per block instead.
+template <typename T, int RP>
+void launch_psweep(int W, size_t lds, hipStream_t s, const T *G, const double *op, const double *r, T *y,
+                   double *x, int nx, int ny, int np, int nr, int dir, SweepSync *sy) {
+  static std::once_flag lds_attr;
+  std::call_once(lds_attr, [] {
+    (void)hipFuncSetAttribute((const void *)k_psweep<T, RP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(2 * 768 * sizeof(double) + 2 * RP * 768 * sizeof(T)));
+  });
+  hipLaunchKernelGGL((k_psweep<T, RP>), dim3(W), dim3(NT), lds, s, G, op, r, y, x, nx, ny, np, nr, dir, sy,
+                     g_spin_cap);
+}
+
+// x += F^-1 r with the persistent sweep kernels (RHS in passes of
+// sweep_rhs<T>()); blockwise = true uses one launch per block instead.
 template <typename T>
 int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const double *r,
                double *x, int nrhs, const SolveWork &sw, hipStream_t s, bool blockwise) {
@@ -1386,20 +1451,26 @@ int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const d
   const int nx = g->nx, ny = g->ny, np = pad_n(ny);
   const int W = (np + 4 * SRW - 1) / (4 * SRW);
   const size_t M = (size_t)nx * ny;
-  static std::once_flag lds_attr;
-  std::call_once(lds_attr, [] {
-    (void)hipFuncSetAttribute((const void *)k_psweep<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(2 * MAXRHS * 768 * sizeof(T)));
-  });
-  for (int q0 = 0; q0 < nrhs; q0 += MAXRHS) {
-    const int nr = std::min(MAXRHS, nrhs - q0);
+  for (int q0 = 0; q0 < nrhs; q0 += sweep_rhs<T>()) {
+    const int nr = std::min(sweep_rhs<T>(), nrhs - q0);
+    const int rp = nr <= 1 ? 1 : nr <= 2 ? 2 : nr <= 4 ? 4 : 8;
     T *yq = (T *)sw.y + (size_t)2 * q0 * nx * np;
-    const size_t lds = (size_t)2 * nr * np * sizeof(T);
+    const size_t lds = (size_t)2 * np * sizeof(double) + (size_t)2 * rp * pad64(np) * sizeof(T);
     HIP_TRY(hipMemsetAsync(sw.sync, 0, sizeof(SweepSync), s));
-    hipLaunchKernelGGL(k_psweep<T>, dim3(W), dim3(NT), lds, s, (const T *)factor, op, r + 2 * M * q0, yq,
-                       (double *)nullptr, nx, ny, np, nr, 0, sw.sync, g_spin_cap);
-    hipLaunchKernelGGL(k_psweep<T>, dim3(W), dim3(NT), lds, s, (const T *)factor, op, (const double *)nullptr,
-                       yq, x + 2 * M * q0, nx, ny, np, nr, 1, sw.sync, g_spin_cap);
+    for (int dir = 0; dir < 2; ++dir) {
+      const double *rq = dir == 0 ? r + 2 * M * q0 : nullptr;
+      double *xq = dir == 1 ? x + 2 * M * q0 : nullptr;
+      const T *G = (const T *)factor;
+      switch (rp) {
+        case 1: launch_psweep<T, 1>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync); break;
+        case 2: launch_psweep<T, 2>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync); break;
+        case 4: launch_psweep<T, 4>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync); break;
+        default:
+          if constexpr (sweep_rhs<T>() >= 8)
+            launch_psweep<T, 8>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync);
+          break;
+      }
+    }
   }
   HIP_TRY(hipGetLastError());
   return FDFD_OK;

@@ -162,8 +162,11 @@ class ServingEngine:
         return x, rep, reused
 
     # ------------------------------------------------------------------ run
-    def run(self, requests, realtime=True):
-        """Serve requests (sorted by arrival); returns Results in request order."""
+    def run(self, requests, realtime=True, progress=None):
+        """Serve requests (sorted by arrival); returns Results in request order.
+
+        progress(done, total), if given, is called after each completion.
+        """
         reqs = sorted(requests, key=lambda r: r.arrival)
         results = {}
         cond = threading.Condition()
@@ -191,6 +194,8 @@ class ServingEngine:
                 state["used"] -= nbytes
                 state["free_slots"].append(slot)
                 cond.notify_all()
+                if progress is not None:
+                    progress(len(results), len(reqs))
 
         threads = []
         with cond:
