@@ -1171,6 +1171,224 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
   }
 }
 
+// ------------------------------------------------------- x-wrap correction
+// The block sweep factors T, the operator without the two x-wrap couplings
+// E = U C V^T: block 0 couples to block m-1 through XM of block 0, and block
+// m-1 to block 0 through XP of block m-1 (U = [E_0, E_{m-1}],
+// C = diag(XM_0, XP_{m-1}), V = [E_{m-1}, E_0]). Woodbury gives
+//   A^-1 = T^-1 - T^-1 U C K^-1 V^T T^-1,  K = I + V^T T^-1 U C,
+// where V^T T^-1 U = [[T^-1_{m-1,0}, T^-1_{m-1,m-1}], [T^-1_{0,0}, T^-1_{0,m-1}]].
+// T^-1_{m-1,m-1} = G_{m-1}. The other corner blocks come from two backward
+// chains after the factorization (Sigma is the selected-inversion recurrence
+// for the diagonal blocks of T^-1, Z the last block column):
+//   Sigma_k = G_k + G_k XP_k Sigma_{k+1} XM_{k+1} G_k,   Z_k = -G_k XP_k Z_{k+1},
+// both starting from G_{m-1}; T^-1_{m-1,0} follows from T^-1_{0,m-1} by the
+// stretch symmetrizer D (D T is complex symmetric, so
+// (T^-1)_{ji} = D_i (T^-1)_{ij} / D_j). K (2n x 2n, complex FP64) is inverted
+// once per factorization by Gauss-Jordan with partial pivoting and kept after
+// the block inverses.
+
+// C = D + sign * A * (rs . B . cs) on n_p x n_p planar matrices, 32 x 32 output
+// tiles per workgroup, MFMA complex products (rs / cs: optional complex row /
+// column scalings of B from band planes; entries beyond n scale by zero).
+template <typename T> struct GemmJob {
+  const T *A, *B, *D;
+  T *C;
+  const double *rs, *cs;
+  T sign;
+};
+template <typename T> struct GemmBatch {
+  int njobs;
+  GemmJob<T> job[2];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_cgemm(GemmBatch<T> batch, int n, int np) {
+  __shared__ T Ar[TB][TBP], Ai[TB][TBP], Br[TB][TBP], Bi[TB][TBP];
+  const int nt = np / TB;
+  const int jb = blockIdx.x / (nt * nt), t = blockIdx.x % (nt * nt);
+  const int ti = t / nt, tj = t % nt;
+  const GemmJob<T> J = jb == 0 ? batch.job[0] : batch.job[1];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, qi = wv >> 1, qj = wv & 1;
+  const size_t np2 = (size_t)np * np;
+  typename Mfma<T>::v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  for (int kt = 0; kt < nt; ++kt) {
+    for (int e = tid; e < TB * TB; e += NT) {
+      const int a = e / TB, b = e % TB;
+      const size_t ia = (size_t)(ti * TB + a) * np + kt * TB + b;
+      Ar[a][b] = J.A[ia];
+      Ai[a][b] = J.A[np2 + ia];
+      const int br = kt * TB + a, bc = tj * TB + b;
+      const size_t ib = (size_t)br * np + bc;
+      double xr = (double)J.B[ib], xi = (double)J.B[np2 + ib];
+      if (J.rs) {
+        const cd s = br < n ? ld_c(J.rs, br) : cd{0, 0};
+        const double yr = s.r * xr - s.i * xi, yi = s.r * xi + s.i * xr;
+        xr = yr; xi = yi;
+      }
+      if (J.cs) {
+        const cd s = bc < n ? ld_c(J.cs, bc) : cd{0, 0};
+        const double yr = s.r * xr - s.i * xi, yi = s.r * xi + s.i * xr;
+        xr = yr; xi = yi;
+      }
+      Br[a][b] = (T)xr;
+      Bi[a][b] = (T)xi;
+    }
+    __syncthreads();
+    cgemm_quad<T>(cr, ci, Ar, Ai, Br, Bi, qi, qj, lane, J.sign);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int row = ti * TB + qi * 16 + Mfma<T>::crow(lane, q), col = tj * TB + qj * 16 + (lane & 15);
+    const size_t idx = (size_t)row * np + col;
+    T vr = cr[q], vi = ci[q];
+    if (J.D) { vr += J.D[idx]; vi += J.D[np2 + idx]; }
+    J.C[idx] = vr;
+    J.C[np2 + idx] = vi;
+  }
+}
+
+__device__ inline cd stretch_d(const fdfd_grid &g, int i, int j) {
+  const double hx = ((double)g.nx * g.dl) / g.nx, hy = ((double)g.ny * g.dl) / g.ny;
+  return cmul(sfactor(0, i, g.nx, g.npml_x, hx, g.omega, g.L0), sfactor(0, j, g.ny, g.npml_y, hy, g.omega, g.L0));
+}
+
+// K = I + V^T T^-1 U C as a row-major complex FP64 matrix of order 2n
+template <typename T>
+__global__ void __launch_bounds__(NT) k_wrap_capacitance(fdfd_grid g, const double *__restrict__ op,
+                                                         const T *__restrict__ Glast, const T *__restrict__ Sig,
+                                                         const T *__restrict__ Y, int np, double *__restrict__ K) {
+  const int n = g.ny, m = g.nx, N = 2 * n;
+  const int j = blockIdx.x * NT + threadIdx.x, i = blockIdx.y;
+  if (j >= N) return;
+  const size_t np2 = (size_t)np * np, M = (size_t)m * n;
+  const int bi = i / n, bj = j / n, a = i % n, b = j % n;
+  cd v;
+  if (bi == 0 && bj == 0) {   // T^-1_{m-1,0}[a][b] = D_{0,b} T^-1_{0,m-1}[b][a] / D_{m-1,a}
+    const size_t e = (size_t)b * np + a;
+    v = cmul(cd{(double)Y[e], (double)Y[np2 + e]}, cmul(stretch_d(g, 0, b), cinv(stretch_d(g, m - 1, a))));
+  } else {
+    const T *src = bi == 0 ? Glast : (bj == 0 ? Sig : Y);
+    const size_t e = (size_t)a * np + b;
+    v = cd{(double)src[e], (double)src[np2 + e]};
+  }
+  const cd c = bj == 0 ? ld_c(op, FDFD_OP_XM * M + b) : ld_c(op, FDFD_OP_XP * M + (size_t)(m - 1) * n + b);
+  v = cmul(v, c);
+  if (i == j) v.r += 1.0;
+  st_c(K, (size_t)i * N + j, v);
+}
+
+// Gauss-Jordan inverse of K in place with partial pivoting: per column j one
+// single-workgroup pivot launch (row swap, column copy, row scaling) and one
+// update launch over the whole matrix; the column swaps are undone last.
+__global__ void __launch_bounds__(NT) k_kgj_pivot(double *K, int N, int j, double *colj, int *piv,
+                                                  int *status) {
+  __shared__ double bv[NT];
+  __shared__ int bi[NT];
+  const int tid = threadIdx.x;
+  double best = -1;
+  int arg = j;
+  for (int i = j + tid; i < N; i += NT) {
+    const cd z = ld_c(K, (size_t)i * N + j);
+    const double v = fabs(z.r) + fabs(z.i);
+    if (v > best) { best = v; arg = i; }
+  }
+  bv[tid] = best;
+  bi[tid] = arg;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      const double o = bv[tid + s];
+      const int oi = bi[tid + s];
+      if (o > bv[tid] || (o == bv[tid] && oi < bi[tid])) { bv[tid] = o; bi[tid] = oi; }
+    }
+    __syncthreads();
+  }
+  const int p = bi[0];
+  if (tid == 0) {
+    piv[j] = p;
+    if (!(bv[0] > 0) || !isfinite(bv[0])) status[0] = 1;
+  }
+  if (p != j) {
+    for (int c = tid; c < N; c += NT) {
+      const cd x = ld_c(K, (size_t)j * N + c), y = ld_c(K, (size_t)p * N + c);
+      st_c(K, (size_t)j * N + c, y);
+      st_c(K, (size_t)p * N + c, x);
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < N; i += NT) st_c(colj, i, ld_c(K, (size_t)i * N + j));
+  const cd pv = ld_c(K, (size_t)j * N + j);
+  const cd d = (pv.r == 0 && pv.i == 0) ? cd{0, 0} : cinv(pv);
+  __syncthreads();
+  for (int c = tid; c < N; c += NT) {
+    const size_t e = (size_t)j * N + c;
+    st_c(K, e, c == j ? d : cmul(ld_c(K, e), d));
+  }
+}
+
+__global__ void __launch_bounds__(NT) k_kgj_update(double *K, int N, int j, const double *__restrict__ colj) {
+  const int c = blockIdx.x * NT + threadIdx.x, i = blockIdx.y;
+  if (c >= N || i == j) return;
+  const cd f = ld_c(colj, i);
+  const size_t e = (size_t)i * N + c;
+  if (c == j) {
+    const cd d = ld_c(K, (size_t)j * N + j);
+    const cd v = cmul(f, d);
+    st_c(K, e, cd{-v.r, -v.i});
+  } else {
+    st_c(K, e, csub(ld_c(K, e), cmul(f, ld_c(K, (size_t)j * N + c))));
+  }
+}
+
+__global__ void __launch_bounds__(NT) k_kgj_unpermute(double *K, int N, const int *__restrict__ piv) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= N) return;
+  for (int j = N - 1; j >= 0; --j) {
+    const int p = piv[j];
+    if (p != j) {
+      const cd x = ld_c(K, (size_t)i * N + j), y = ld_c(K, (size_t)i * N + p);
+      st_c(K, (size_t)i * N + j, y);
+      st_c(K, (size_t)i * N + p, x);
+    }
+  }
+}
+
+// f = -U C K^-1 V^T t for every right-hand side: one wave per row of K^-1;
+// rows < n land in block 0 (times XM_0), the others in block m-1 (times
+// XP_{m-1}). f must be zero elsewhere.
+__global__ void __launch_bounds__(NT) k_wrap_rhs(fdfd_grid g, const double *__restrict__ op,
+                                                 const double *__restrict__ Kinv, const double *__restrict__ t,
+                                                 double *__restrict__ f) {
+  const int n = g.ny, m = g.nx, N = 2 * n;
+  const int lane = threadIdx.x & 63, i = blockIdx.x * (NT / 64) + (threadIdx.x >> 6), q = blockIdx.y;
+  if (i >= N) return;
+  const size_t M = (size_t)m * n;
+  const double *tq = t + 2 * M * q;
+  double sr = 0, si = 0;
+  for (int j = lane; j < N; j += 64) {
+    const cd kv = ld_c(Kinv, (size_t)i * N + j);
+    const cd sv = j < n ? ld_c(tq, (size_t)(m - 1) * n + j) : ld_c(tq, (size_t)(j - n));
+    sr += kv.r * sv.r - kv.i * sv.i;
+    si += kv.r * sv.i + kv.i * sv.r;
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    sr += __shfl_xor(sr, off);
+    si += __shfl_xor(si, off);
+  }
+  if (lane == 0) {
+    const size_t pos = i < n ? (size_t)i : (size_t)(m - 1) * n + (i - n);
+    const cd c = i < n ? ld_c(op, FDFD_OP_XM * M + i) : ld_c(op, FDFD_OP_XP * M + (size_t)(m - 1) * n + (i - n));
+    const cd w = cmul(c, cd{sr, si});
+    st_c(f + 2 * M * q, pos, cd{-w.r, -w.i});
+  }
+}
+
+__global__ void __launch_bounds__(NT) k_axpy(size_t len, const double *__restrict__ a, double *__restrict__ y) {
+  for (size_t e = (size_t)blockIdx.x * NT + threadIdx.x; e < len; e += (size_t)gridDim.x * NT) y[e] += a[e];
+}
+
 // ------------------------------------------------------------------ norms
 // part[r][blk] = sum |w v|^2 over a slice; w = stretch weight when weighted.
 __device__ inline double stretch_abs2(const fdfd_grid &g, int i, int j) {
@@ -1337,6 +1555,7 @@ size_t factor_work_bytes(const fdfd_grid *g, int precision) {
 
 struct SolveWork {
   double *r; void *y; double *part; double *norms; SweepSync *sync;
+  double *t, *f;   // wrap correction: T^-1 r and -U C K^-1 V^T t
 };
 SolveWork solve_work_layout(const fdfd_grid *g, int precision, int nrhs, void *work) {
   const size_t M = (size_t)g->nx * g->ny, np = pad_n(g->ny), es = elem_size(precision);
@@ -1347,8 +1566,30 @@ SolveWork solve_work_layout(const fdfd_grid *g, int precision, int nrhs, void *w
   sw.y = base + off;             off += align_up(2 * es * (size_t)g->nx * np * nrhs, 256);
   sw.part = (double *)(base + off); off += align_up(8 * (size_t)NORM_PARTS * nrhs * 2, 256);
   sw.norms = (double *)(base + off); off += align_up(8 * 4 * (size_t)nrhs, 256);
-  sw.sync = (SweepSync *)(base + off);
+  sw.sync = (SweepSync *)(base + off); off += align_up(sizeof(SweepSync), 256);
+  sw.t = (double *)(base + off);  off += align_up(16 * M * nrhs, 256);
+  sw.f = (double *)(base + off);
   return sw;
+}
+size_t solve_work_bytes(const fdfd_grid *g, int precision, int nrhs) {
+  const size_t M = (size_t)g->nx * g->ny, np = pad_n(g->ny), es = elem_size(precision);
+  return align_up(16 * M * nrhs, 256) + align_up(2 * es * (size_t)g->nx * np * nrhs, 256) +
+         align_up(8 * (size_t)NORM_PARTS * nrhs * 2, 256) + align_up(8 * 4 * (size_t)nrhs, 256) +
+         align_up(sizeof(SweepSync), 256) + 2 * align_up(16 * M * nrhs, 256);
+}
+
+// The factor buffer: the block inverses G_k ([m][2][np][np] of T), then
+// K^-1 of the wrap correction (row-major complex FP64, order 2n).
+size_t factor_blocks_bytes(const fdfd_grid *g, int precision) {
+  const size_t np = pad_n(g->ny);
+  return align_up((size_t)g->nx * 2 * np * np * elem_size(precision), 256);
+}
+size_t factor_total_bytes(const fdfd_grid *g, int precision) {
+  const size_t N = 2 * (size_t)g->ny;
+  return factor_blocks_bytes(g, precision) + 16 * N * N;
+}
+inline double *wrap_kinv(const fdfd_grid *g, int precision, const void *factor) {
+  return (double *)((char *)factor + factor_blocks_bytes(g, precision));
 }
 
 template <typename T, bool PIV>
@@ -1430,6 +1671,60 @@ int sweep_blockwise(const fdfd_grid *g, const double *op, const void *factor, co
   return FDFD_OK;
 }
 
+// The wrap correction of a finished block factorization: the two backward
+// chains (two launches per block), K, and its inverse (2 launches per column
+// of K). Temporaries live in the factorization workspace, whose exchange
+// buffers are free once the block inverses are complete.
+template <typename T>
+int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, hipStream_t s) {
+  const int prec = sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64;
+  const int m = g->nx, n = g->ny, np = pad_n(n), nt = np / TB, N = 2 * n;
+  const size_t np2 = (size_t)np * np, M = (size_t)m * n;
+  FactorWork fw = factor_work_layout(g, prec, work);
+  const T *G = (const T *)factor;
+  T *Sig = (T *)fw.W, *Q = (T *)fw.W + 2 * np2;
+  T *Z[2] = {(T *)fw.xcol, (T *)fw.xcol + 2 * np2};
+  const T *Glast = G + (size_t)(m - 1) * 2 * np2;
+  const T *sig_prev = Glast, *z_prev = Glast;
+  int zi = 0;
+  for (int k = m - 2; k >= 0; --k) {
+    const T *Gk = G + (size_t)k * 2 * np2;
+    GemmBatch<T> b1;
+    memset(&b1, 0, sizeof b1);
+    b1.njobs = 2;
+    // Q = G_k (XP_k Sigma_{k+1} XM_{k+1})
+    b1.job[0] = GemmJob<T>{Gk, sig_prev, nullptr, Q, op + 2 * (FDFD_OP_XP * M + (size_t)k * n),
+                           op + 2 * (FDFD_OP_XM * M + (size_t)(k + 1) * n), T(1)};
+    // Z_k = -G_k (XP_k Z_{k+1})
+    b1.job[1] = GemmJob<T>{Gk, z_prev, nullptr, Z[zi], op + 2 * (FDFD_OP_XP * M + (size_t)k * n), nullptr,
+                           T(-1)};
+    hipLaunchKernelGGL(k_cgemm<T>, dim3(2 * nt * nt), dim3(NT), 0, s, b1, n, np);
+    // Sigma_k = G_k + Q G_k
+    GemmBatch<T> b2;
+    memset(&b2, 0, sizeof b2);
+    b2.njobs = 1;
+    b2.job[0] = GemmJob<T>{Q, Gk, Gk, Sig, nullptr, nullptr, T(1)};
+    hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(NT), 0, s, b2, n, np);
+    sig_prev = Sig;
+    z_prev = Z[zi];
+    zi ^= 1;
+  }
+  HIP_TRY(hipGetLastError());
+  double *K = wrap_kinv(g, prec, factor);
+  hipLaunchKernelGGL(k_wrap_capacitance<T>, dim3((N + NT - 1) / NT, N), dim3(NT), 0, s, *g, op, Glast, sig_prev,
+                     z_prev, np, K);
+  // Gauss-Jordan scratch in the (now free) pivot-inverse area
+  double *colj = (double *)fw.pinv_all;
+  int *piv = (int *)(colj + 2 * (size_t)N);
+  for (int j = 0; j < N; ++j) {
+    hipLaunchKernelGGL(k_kgj_pivot, dim3(1), dim3(NT), 0, s, K, N, j, colj, piv, fw.status);
+    hipLaunchKernelGGL(k_kgj_update, dim3((N + NT - 1) / NT, N), dim3(NT), 0, s, K, N, j, (const double *)colj);
+  }
+  hipLaunchKernelGGL(k_kgj_unpermute, dim3((N + NT - 1) / NT), dim3(NT), 0, s, K, N, (const int *)piv);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
 template <typename T, int RP>
 void launch_psweep(int W, size_t lds, hipStream_t s, const T *G, const double *op, const double *r, T *y,
                    double *x, int nx, int ny, int np, int nr, int dir, SweepSync *sy) {
@@ -1456,7 +1751,9 @@ int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const d
     const int rp = nr <= 1 ? 1 : nr <= 2 ? 2 : nr <= 4 ? 4 : 8;
     T *yq = (T *)sw.y + (size_t)2 * q0 * nx * np;
     const size_t lds = (size_t)2 * np * sizeof(double) + (size_t)2 * rp * pad64(np) * sizeof(T);
-    HIP_TRY(hipMemsetAsync(sw.sync, 0, sizeof(SweepSync), s));
+    // step flags start at zero; the abort flag is cleared by the caller, so
+    // an abort stays visible across the sweeps of one solve attempt
+    HIP_TRY(hipMemsetAsync(sw.sync->done, 0, sizeof sw.sync->done, s));
     for (int dir = 0; dir < 2; ++dir) {
       const double *rq = dir == 0 ? r + 2 * M * q0 : nullptr;
       double *xq = dir == 1 ? x + 2 * M * q0 : nullptr;
@@ -1626,8 +1923,7 @@ size_t fdfd_op_bytes(const fdfd_grid *g) {
 }
 size_t fdfd_factor_bytes(const fdfd_grid *g, int precision) {
   if (check_grid(g) || check_prec(precision)) return 0;
-  const size_t np = pad_n(g->ny);
-  return (size_t)g->nx * 2 * np * np * elem_size(precision);
+  return factor_total_bytes(g, precision);
 }
 size_t fdfd_factor_workspace_bytes(const fdfd_grid *g, int precision) {
   if (check_grid(g) || check_prec(precision)) return 0;
@@ -1635,10 +1931,7 @@ size_t fdfd_factor_workspace_bytes(const fdfd_grid *g, int precision) {
 }
 size_t fdfd_solve_workspace_bytes(const fdfd_grid *g, int precision, int nrhs) {
   if (check_grid(g) || check_prec(precision) || nrhs < 1) return 0;
-  const size_t M = (size_t)g->nx * g->ny, np = pad_n(g->ny), es = elem_size(precision);
-  return align_up(16 * M * nrhs, 256) + align_up(2 * es * (size_t)g->nx * np * nrhs, 256) +
-         align_up(8 * (size_t)NORM_PARTS * nrhs * 2, 256) + align_up(8 * 4 * (size_t)nrhs, 256) +
-         align_up(sizeof(SweepSync), 256);
+  return solve_work_bytes(g, precision, nrhs);
 }
 
 int fdfd_assemble(const fdfd_grid *g, const double *eps, double *op, hipStream_t stream) {
@@ -1676,20 +1969,31 @@ int fdfd_factor(const fdfd_grid *g, const double *op, void *factor, int precisio
   if (!op || !factor || !work) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_factor");
   const bool piv = precision & FDFD_FACTOR_PIVOT;
   const bool fp64 = base_prec(precision) == FDFD_PREC_FP64;
+  bool ran = false;
   if (!(precision & FDFD_FACTOR_STEPWISE)) {
-    bool ran = false;
     int rc = fp64 ? (piv ? factor_persistent<double, true>(g, op, factor, work, stream, &ran)
                          : factor_persistent<double, false>(g, op, factor, work, stream, &ran))
                   : (piv ? factor_persistent<float, true>(g, op, factor, work, stream, &ran)
                          : factor_persistent<float, false>(g, op, factor, work, stream, &ran));
     if (rc) return rc;
-    if (ran) return FDFD_OK;
   }
-  if (fp64)
-    return piv ? factor_impl<double, true>(g, op, factor, work, stream)
-               : factor_impl<double, false>(g, op, factor, work, stream);
-  return piv ? factor_impl<float, true>(g, op, factor, work, stream)
-             : factor_impl<float, false>(g, op, factor, work, stream);
+  if (!ran) {
+    int rc = fp64 ? (piv ? factor_impl<double, true>(g, op, factor, work, stream)
+                         : factor_impl<double, false>(g, op, factor, work, stream))
+                  : (piv ? factor_impl<float, true>(g, op, factor, work, stream)
+                         : factor_impl<float, false>(g, op, factor, work, stream));
+    if (rc) return rc;
+  }
+  return fp64 ? wrap_setup<double>(g, op, factor, work, stream) : wrap_setup<float>(g, op, factor, work, stream);
+}
+
+int fdfd_factor_wrap(const fdfd_grid *g, const double *op, void *factor, int precision, void *work,
+                     hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (int rc = check_prec(precision)) return rc;
+  if (!op || !factor || !work) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_factor_wrap");
+  return base_prec(precision) == FDFD_PREC_FP64 ? wrap_setup<double>(g, op, factor, work, stream)
+                                                : wrap_setup<float>(g, op, factor, work, stream);
 }
 
 int fdfd_factor_status(const fdfd_grid *g, int precision, void *work, hipStream_t stream) {
@@ -1709,6 +2013,7 @@ int fdfd_sweep(const fdfd_grid *g, const double *op, const void *factor, int pre
   if (int rc = check_prec(precision)) return rc;
   if (!op || !factor || !r || !x || !work || nrhs < 1) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_sweep");
   SolveWork sw = solve_work_layout(g, precision, nrhs, work);
+  HIP_TRY(hipMemsetAsync(sw.sync, 0, sizeof(SweepSync), stream));
   int rc = base_prec(precision) == FDFD_PREC_FP64
                ? sweep_impl<double>(g, op, factor, r, x, nrhs, sw, stream, false)
                : sweep_impl<float>(g, op, factor, r, x, nrhs, sw, stream, false);
@@ -1774,17 +2079,30 @@ int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int pre
   bool blockwise = false;
   for (int attempt = 0; attempt < 2; ++attempt) {
     memset(&st, 0, sizeof st);
+    HIP_TRY(hipMemsetAsync(sw.sync, 0, sizeof(SweepSync), stream));
     HIP_TRY(hipMemsetAsync(x, 0, 16 * M * nrhs, stream));
     HIP_TRY(hipMemcpyAsync(rr, b, 16 * M * nrhs, hipMemcpyDeviceToDevice, stream));
     if (transpose && (rc = fdfd_stretch_scale(g, rr, nrhs, 1, stream))) return rc;
     std::vector<double> prev(nrhs, INFINITY);
     int stall_count = 0;
     bool aborted = false;
+    const bool fp64 = base_prec(precision) == FDFD_PREC_FP64;
+    const double *kinv = wrap_kinv(g, precision, factor);
     for (int it = 1; it <= max_iter; ++it) {
-      rc = base_prec(precision) == FDFD_PREC_FP64
-               ? sweep_impl<double>(g, op, factor, rr, x, nrhs, sw, stream, blockwise)
-               : sweep_impl<float>(g, op, factor, rr, x, nrhs, sw, stream, blockwise);
+      // t = T^-1 r; f = -U C K^-1 V^T t; t += T^-1 f; x += t  (x += A^-1 r)
+      HIP_TRY(hipMemsetAsync(sw.t, 0, 16 * M * nrhs, stream));
+      rc = fp64 ? sweep_impl<double>(g, op, factor, rr, sw.t, nrhs, sw, stream, blockwise)
+                : sweep_impl<float>(g, op, factor, rr, sw.t, nrhs, sw, stream, blockwise);
       if (rc) return rc;
+      HIP_TRY(hipMemsetAsync(sw.f, 0, 16 * M * nrhs, stream));
+      hipLaunchKernelGGL(k_wrap_rhs, dim3((2 * g->ny + NT / 64 - 1) / (NT / 64), nrhs), dim3(NT), 0, stream, *g,
+                         op, kinv, (const double *)sw.t, sw.f);
+      HIP_TRY(hipGetLastError());
+      rc = fp64 ? sweep_impl<double>(g, op, factor, sw.f, sw.t, nrhs, sw, stream, blockwise)
+                : sweep_impl<float>(g, op, factor, sw.f, sw.t, nrhs, sw, stream, blockwise);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_axpy, dim3(1024), dim3(NT), 0, stream, 2 * M * nrhs, (const double *)sw.t, x);
+      HIP_TRY(hipGetLastError());
       st.iterations = it;
       // r = b' - A x, with b' = b or D^-1 b
       HIP_TRY(hipMemcpyAsync(rr, b, 16 * M * nrhs, hipMemcpyDeviceToDevice, stream));

@@ -214,6 +214,8 @@ class DeviceOperator:
         if self.engine is not None:
             torch.cuda.current_stream(self.device).synchronize()   # operator complete
             self.engine.wait(self.engine.submit(self.grid, self.op, fac, p, work))
+            L.check(self.lib.fdfd_factor_wrap(g, _vp(self.op), _vp(fac), p, _vp(work), s))
+            L.check(self.lib.fdfd_factor_status(g, p, _vp(work), s))
         else:
             L.check(self.lib.fdfd_factor(g, _vp(self.op), _vp(fac), p, _vp(work), s))
             L.check(self.lib.fdfd_factor_status(g, p, _vp(work), s))

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define FDFD_ABI_VERSION 1
+#define FDFD_ABI_VERSION 2
 
 enum {
   FDFD_OK = 0,
@@ -105,26 +105,36 @@ int fdfd_residual(const fdfd_grid *g, const double *op, const double *b,
                   const double *x, double *r, int nrhs, hipStream_t stream);
 
 /* Replaces pardisoSolver(A, mtype=13).factor() (linalg.py:135-136): block
- * sweep factorization of the x-non-periodic part of A. factor holds
- * fdfd_factor_bytes, work fdfd_factor_workspace_bytes. Uses one persistent
- * launch (and waits for it) when the tile grid fits the device, otherwise
- * enqueues one launch per Gauss-Jordan step; the numeric status is read by
- * fdfd_factor_status. */
+ * sweep factorization of T, the x-non-periodic part of A, followed by the
+ * exact correction for the two x-wrap couplings (Woodbury: the inverse of a
+ * 2ny x 2ny capacitance matrix built from corner blocks of T^-1). factor
+ * holds fdfd_factor_bytes, work fdfd_factor_workspace_bytes. The block
+ * factorization uses one persistent launch (and waits for it) when the tile
+ * grid fits the device, otherwise one launch per Gauss-Jordan step; the
+ * numeric status is read by fdfd_factor_status. */
 int fdfd_factor(const fdfd_grid *g, const double *op, void *factor, int precision,
                 void *work, hipStream_t stream);
+
+/* The x-wrap correction step of fdfd_factor alone, for a block
+ * factorization completed by the batching engine (same arguments; enqueued
+ * on stream). No reference counterpart: part of the factor() replacement. */
+int fdfd_factor_wrap(const fdfd_grid *g, const double *op, void *factor, int precision,
+                     void *work, hipStream_t stream);
 
 /* Reads the numeric status the factor kernels left in work (waits for the
  * stream): FDFD_OK or FDFD_ERR_NUMERIC. */
 int fdfd_factor_status(const fdfd_grid *g, int precision, void *work, hipStream_t stream);
 
-/* One application of the factor: x += F^-1 r (nrhs vectors). */
+/* One block sweep with the factor: x += T^-1 r (nrhs vectors), without the
+ * x-wrap correction that fdfd_solve applies. */
 int fdfd_sweep(const fdfd_grid *g, const double *op, const void *factor, int precision,
                const double *r, double *x, int nrhs, void *work, hipStream_t stream);
 
 /* Replaces pardisoSolver.solve(b) (linalg.py:137) and spsolve (linalg.py:141):
  * solves A x = b (transpose = 0) or A^T x = b (transpose = 1, the adjoint
- * system) to ||b - A x|| <= tol ||b|| per RHS by refinement with the factor,
- * at most max_iter sweeps. An all-zero b gives x = 0 without a sweep
+ * system) to ||b - A x|| <= tol ||b|| per RHS by refinement with the
+ * wrap-corrected factor (each iteration: two block sweeps and the
+ * capacitance product), at most max_iter iterations. An all-zero b gives x = 0 without a sweep
  * (linalg.py:128-129). rel_res (optional, device-independent host array of
  * nrhs doubles) receives the final relative residuals. */
 int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int precision,
@@ -169,8 +179,9 @@ int fdfd_fields(const fdfd_grid *g, const double *eps, const double *x, double *
  * new ones join at the next launch. Arguments of fdfd_engine_submit are as
  * for fdfd_factor and must stay valid until fdfd_engine_wait returns; the
  * operator must be complete on the device before submission.
- * fdfd_engine_wait blocks until the factor is complete and returns its
- * numeric status (as fdfd_factor_status). */
+ * fdfd_engine_wait blocks until the block factorization is complete and
+ * returns its numeric status (as fdfd_factor_status); fdfd_factor_wrap then
+ * completes the factor. */
 typedef struct fdfd_engine fdfd_engine;
 fdfd_engine *fdfd_engine_create(int device);
 int fdfd_engine_destroy(fdfd_engine *engine);

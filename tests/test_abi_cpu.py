@@ -25,7 +25,8 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
         assert n in L.SIGNATURES, f"{n} has no ctypes signature"
-    assert lib.fdfd_abi_version() == 1
+    version = int(re.search(r"#define FDFD_ABI_VERSION (\d+)", open(HEADER).read()).group(1))
+    assert lib.fdfd_abi_version() == version
 
 
 def test_sizes():
@@ -33,11 +34,13 @@ def test_sizes():
     g = L.FdfdGrid(512, 512, 20, 20, 0, 0, 1.2e15, 0.02, 1e-6)
     assert lib.fdfd_block_padding() == 32
     assert lib.fdfd_op_bytes(ctypes.byref(g)) == 5 * 512 * 512 * 16
-    assert lib.fdfd_factor_bytes(ctypes.byref(g), 0) == 512 * 2 * 512 * 512 * 8
-    assert lib.fdfd_factor_bytes(ctypes.byref(g), 1) == 512 * 2 * 512 * 512 * 4
+    # block inverses, then the complex FP64 inverse of the 2n x 2n wrap capacitance matrix
+    kinv = 16 * 1024 * 1024
+    assert lib.fdfd_factor_bytes(ctypes.byref(g), 0) == 512 * 2 * 512 * 512 * 8 + kinv
+    assert lib.fdfd_factor_bytes(ctypes.byref(g), 1) == 512 * 2 * 512 * 512 * 4 + kinv
     # block size padded to the tile edge
     g2 = L.FdfdGrid(300, 300, 20, 20, 0, 0, 1.2e15, 0.02, 1e-6)
-    assert lib.fdfd_factor_bytes(ctypes.byref(g2), 0) == 300 * 2 * 320 * 320 * 8
+    assert lib.fdfd_factor_bytes(ctypes.byref(g2), 0) == 300 * 2 * 320 * 320 * 8 + 16 * 600 * 600
     assert lib.fdfd_solve_workspace_bytes(ctypes.byref(g2), 0, 4) > 16 * 300 * 300 * 4
 
 
