@@ -1187,6 +1187,15 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
 // (T^-1)_{ji} = D_i (T^-1)_{ij} / D_j). K (2n x 2n, complex FP64) is inverted
 // once per factorization by Gauss-Jordan with partial pivoting and kept after
 // the block inverses.
+// With PMLs of at least WRAP_MIN_PML cells on both axes the far corners
+// T^-1_{0,m-1} and T^-1_{m-1,0} (waves crossing the whole domain and both
+// x-PMLs) are dropped and the Sigma chain starts at block
+// L = 2 NPML_x + WRAP_EXTRA with Sigma_L = G_L (the part of T^-1_{0,0} that
+// returns from beyond L has crossed the x-PML twice). K is then approximate
+// to about 1e-9 and a FP64 solve takes two refinement steps instead of one,
+// while the chains shrink from 3 (m - 1) products to 2 L. Without such PMLs
+// (periodic axes, where waves along the blocks are never absorbed) the
+// chains run over all blocks and K is exact.
 
 // C = D + sign * A * (rs . B . cs) on n_p x n_p planar matrices, 32 x 32 output
 // tiles per workgroup, MFMA complex products (rs / cs: optional complex row /
@@ -1255,6 +1264,9 @@ __device__ inline cd stretch_d(const fdfd_grid &g, int i, int j) {
 }
 
 // K = I + V^T T^-1 U C as a row-major complex FP64 matrix of order 2n
+constexpr int WRAP_MIN_PML = 4, WRAP_EXTRA = 16;
+
+// Y = NULL: the far corners are taken as zero
 template <typename T>
 __global__ void __launch_bounds__(NT) k_wrap_capacitance(fdfd_grid g, const double *__restrict__ op,
                                                          const T *__restrict__ Glast, const T *__restrict__ Sig,
@@ -1264,8 +1276,10 @@ __global__ void __launch_bounds__(NT) k_wrap_capacitance(fdfd_grid g, const doub
   if (j >= N) return;
   const size_t np2 = (size_t)np * np, M = (size_t)m * n;
   const int bi = i / n, bj = j / n, a = i % n, b = j % n;
-  cd v;
-  if (bi == 0 && bj == 0) {   // T^-1_{m-1,0}[a][b] = D_{0,b} T^-1_{0,m-1}[b][a] / D_{m-1,a}
+  cd v = {0, 0};
+  if (bi == bj && !Y) {
+    // far corner dropped
+  } else if (bi == 0 && bj == 0) {   // T^-1_{m-1,0}[a][b] = D_{0,b} T^-1_{0,m-1}[b][a] / D_{m-1,a}
     const size_t e = (size_t)b * np + a;
     v = cmul(cd{(double)Y[e], (double)Y[np2 + e]}, cmul(stretch_d(g, 0, b), cinv(stretch_d(g, m - 1, a))));
   } else {
@@ -1685,20 +1699,24 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, h
   T *Sig = (T *)fw.W, *Q = (T *)fw.W + 2 * np2;
   T *Z[2] = {(T *)fw.xcol, (T *)fw.xcol + 2 * np2};
   const T *Glast = G + (size_t)(m - 1) * 2 * np2;
-  const T *sig_prev = Glast, *z_prev = Glast;
+  // exact chains over all blocks, or the truncated Sigma chain without Z
+  const bool exact = g->npml_x < WRAP_MIN_PML || g->npml_y < WRAP_MIN_PML || 2 * g->npml_x + WRAP_EXTRA >= m - 1;
+  const int L = exact ? m - 1 : 2 * g->npml_x + WRAP_EXTRA;
+  const T *sig_prev = G + (size_t)L * 2 * np2, *z_prev = exact ? Glast : nullptr;
   int zi = 0;
-  for (int k = m - 2; k >= 0; --k) {
+  for (int k = L - 1; k >= 0; --k) {
     const T *Gk = G + (size_t)k * 2 * np2;
     GemmBatch<T> b1;
     memset(&b1, 0, sizeof b1);
-    b1.njobs = 2;
+    b1.njobs = exact ? 2 : 1;
     // Q = G_k (XP_k Sigma_{k+1} XM_{k+1})
     b1.job[0] = GemmJob<T>{Gk, sig_prev, nullptr, Q, op + 2 * (FDFD_OP_XP * M + (size_t)k * n),
                            op + 2 * (FDFD_OP_XM * M + (size_t)(k + 1) * n), T(1)};
     // Z_k = -G_k (XP_k Z_{k+1})
-    b1.job[1] = GemmJob<T>{Gk, z_prev, nullptr, Z[zi], op + 2 * (FDFD_OP_XP * M + (size_t)k * n), nullptr,
-                           T(-1)};
-    hipLaunchKernelGGL(k_cgemm<T>, dim3(2 * nt * nt), dim3(NT), 0, s, b1, n, np);
+    if (exact)
+      b1.job[1] = GemmJob<T>{Gk, z_prev, nullptr, Z[zi], op + 2 * (FDFD_OP_XP * M + (size_t)k * n), nullptr,
+                             T(-1)};
+    hipLaunchKernelGGL(k_cgemm<T>, dim3(b1.njobs * nt * nt), dim3(NT), 0, s, b1, n, np);
     // Sigma_k = G_k + Q G_k
     GemmBatch<T> b2;
     memset(&b2, 0, sizeof b2);
@@ -1706,8 +1724,10 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, h
     b2.job[0] = GemmJob<T>{Q, Gk, Gk, Sig, nullptr, nullptr, T(1)};
     hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(NT), 0, s, b2, n, np);
     sig_prev = Sig;
-    z_prev = Z[zi];
-    zi ^= 1;
+    if (exact) {
+      z_prev = Z[zi];
+      zi ^= 1;
+    }
   }
   HIP_TRY(hipGetLastError());
   double *K = wrap_kinv(g, prec, factor);

@@ -75,6 +75,27 @@ def test_solve_matches_oracle(pol, N, npml, precision):
         assert relerr(x[q].cpu().numpy(), xref) < 1e-10
 
 
+@pytest.mark.parametrize("npml,precision,max_steps", [
+    ([15, 15], "fp64", 2), ([15, 15], "c64", 4), ([0, 12], "fp64", 1), ([0, 12], "c64", 4),
+    ([12, 0], "fp64", 1),
+])
+def test_wrap_correction_refinement_steps(npml, precision, max_steps):
+    """With the x-wrap correction a FP64 solve needs two refinement steps
+    (one when the correction is exact, with a periodic axis) and a complex64
+    solve at most four; without it the wrap costs tens of sweeps."""
+    p = C.c3_request(3, n=128)
+    N = p.shape
+    op = DeviceOperator(N, p.omega, p.dl, npml, "Ez", p.L0, DEV)
+    op.set_eps(p.eps)
+    b = p.rhs()
+    x, rep = op.solve(torch.tensor(b, device=DEV), precision=precision, tol=1e-10)
+    assert rep.converged and not rep.fell_back
+    assert rep.iterations <= max_steps, rep.iterations
+    sim = OracleSimulation(p.omega, p.eps, p.dl, npml, "Ez")
+    for q in range(b.shape[0]):
+        assert relerr(x[q].cpu().numpy(), solver_direct(sim.A, b[q]).reshape(N)) < 1e-8
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_c1_simulation_matches_golden_and_oracle(precision):
     p = C.c1_point()
