@@ -43,9 +43,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1, help="untimed batches (of 8 requests each)")
     ap.add_argument("--batch", type=int, default=64, help="requests per step")
     ap.add_argument("--rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
-    ap.add_argument("--slots", type=int, default=4, help="requests in flight")
+    ap.add_argument("--slots", type=int, default=16, help="requests in flight")
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="budget of the timed CPU sample (sizes beyond it are extrapolated)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--roofline-only", action="store_true", help="only the dominant-kernel measurement")
@@ -112,26 +113,49 @@ def copy_calibration(nbytes=256 << 20, reps=3):
 
 
 def cpu_baseline(reqs, seconds):
-    """Oracle (CSR + SuperLU, refactored per RHS as linalg.py:122-149 does) on a prefix of the stream."""
+    """The oracle's restatement of the reference 'scipy' path (CSR assembly +
+    SuperLU spsolve, refactored per right-hand side as linalg.py:122-149
+    does), one host core, on a bounded sample of the stream: one solve of the
+    first request of each grid size, smallest first, while the sample stays
+    within `seconds`; sizes left out are extrapolated with the growth
+    exponent of the two largest timed sizes. The stream's CPU rate is its
+    solves divided by the sum of per-solve times over its requests."""
+    import math
     from threadpoolctl import threadpool_limits
     from oracle import construct_A, solver_direct
-    done, rhs, sizes, t0 = 0, 0, [], time.perf_counter()
+    first = {}
+    for r in reqs:
+        first.setdefault(r.problem.shape[0], r.problem)
+    sizes = sorted(first)
+    per_solve, spent = {}, 0.0
     with threadpool_limits(limits=1):
-        for r in reqs:
-            p = r.problem
-            N = p.shape
-            for q in range(p.srcs.shape[0]):
-                A, _ = construct_A(p.omega, [0, N[0] * p.dl], [0, N[1] * p.dl], p.eps, p.npml, p.pol, p.L0)
-                solver_direct(A, p.srcs[q] * 1j * p.omega)
-                rhs += 1
-            done += 1
-            sizes.append(N[0])
-            if time.perf_counter() - t0 >= seconds:
+        for n in sizes:
+            if per_solve and spent >= seconds:
                 break
-    wall = time.perf_counter() - t0
-    return {"value": round(rhs / wall, 4), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"first {done} requests of the stream (grids {sizes}), {rhs} solves, "
-                      f"assembly + SuperLU spsolve per solve, {wall:.1f} s on 1 host core"}
+            p = first[n]
+            t = time.perf_counter()
+            A, _ = construct_A(p.omega, [0, n * p.dl], [0, p.shape[1] * p.dl], p.eps, p.npml, p.pol, p.L0)
+            solver_direct(A, p.srcs[0] * 1j * p.omega)
+            per_solve[n] = time.perf_counter() - t
+            spent += per_solve[n]
+            log(f"  cpu: {n}x{n} solve {per_solve[n]:.2f} s")
+    timed = sorted(per_solve)
+    extrap = [n for n in sizes if n not in per_solve]
+    if extrap:
+        if len(timed) >= 2:
+            a, b = timed[-2], timed[-1]
+            expo = math.log(per_solve[b] / per_solve[a]) / math.log(b / a)
+        else:
+            expo = 3.0
+        for n in extrap:
+            per_solve[n] = per_solve[timed[-1]] * (n / timed[-1]) ** expo
+    total = sum(r.problem.srcs.shape[0] * per_solve[r.problem.shape[0]] for r in reqs)
+    nsolves = sum(r.problem.srcs.shape[0] for r in reqs)
+    note = (f"; {extrap} extrapolated with exponent {expo:.2f} from {timed[-2:]}" if extrap else "")
+    return {"value": round(nsolves / total, 5), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"one timed solve per grid size {timed} ({spent:.1f} s on 1 host core){note}; "
+                      f"rate = {nsolves} stream solves / sum of per-solve times",
+            "per_solve_s": {str(n): round(per_solve[n], 3) for n in sizes}}
 
 
 def run_stream(reqs, precision, slots, tol):

@@ -32,7 +32,7 @@ namespace {
 constexpr int TB = 32;        // pivot tile / output tile edge
 constexpr int TBP = TB + 1;   // padded LDS row
 constexpr int NT = 256;       // threads per workgroup (4 waves)
-constexpr int MAXJ = 8;       // jobs per batched step launch
+constexpr int MAXJ = 16;      // jobs per batched step launch
 constexpr int MAXR = 4;       // right-hand sides per sweep pass
 constexpr int NORM_PARTS = 512;
 
@@ -299,11 +299,32 @@ template <typename T> struct FJob {
   int m, n, np, nt;  // blocks, block size, padded block size, tiles per edge
   int k, p;          // step performed by this launch; k < 0: initial pivot
 };
+// Workgroup b < njobs takes the critical tile of job b (the tile inverted for
+// the next step), so those start first; the other tiles of job j are
+// workgroups [wg_start[j], wg_start[j + 1]), with wg_start[0] = njobs.
 template <typename T> struct FBatch {
   int njobs;
   int wg_start[MAXJ + 1];
   FJob<T> job[MAXJ];
 };
+// Tiles per launch of a job, and its critical tile: (p+1, p+1), or (0, 0)
+// at the last step of a block (and the only tile of the initial pivot).
+template <typename T>
+__host__ __device__ inline int step_tiles(const FJob<T> &J) { return J.k < 0 ? 1 : J.nt * J.nt; }
+template <typename T>
+__host__ __device__ inline int critical_tile(const FJob<T> &J) {
+  return (J.k >= 0 && J.p + 1 < J.nt) ? (J.p + 1) * (J.nt + 1) : 0;
+}
+template <typename T>
+inline int batch_layout(FBatch<T> &b) {
+  int wg = b.njobs;
+  for (int j = 0; j < b.njobs; ++j) {
+    b.wg_start[j] = wg;
+    wg += step_tiles(b.job[j]) - 1;
+  }
+  b.wg_start[b.njobs] = wg;
+  return wg;
+}
 
 // Element (a, b) of the working matrix of block k before step p. For p == 0
 // this is S_k = T_k - l_k G_{k-1} u_{k-1} (identity on the padding).
@@ -400,10 +421,13 @@ __device__ inline void fast_crecip(T zr, T zi, T &outr, T &outi) {
 }
 
 // Inverse of the TB x TB tile in (Ar, Ai), in place, by Gauss-Jordan with
-// diagonal pivots (the default; the tile is a diagonal block of a Schur
-// complement of the PML-damped operator). Thread t keeps row t/8, columns
-// 4(t%8)..+3 in registers and publishes them each step to one of two LDS
-// buffers (Pr/Pi, Rr/Ri), so one barrier per pivot orders the exchange.
+// 2 x 2 diagonal block pivots (the default; the tile is a diagonal block of
+// a Schur complement of the PML-damped operator): TB / 2 steps, each inverting the pivot block
+// in closed form and applying a rank-2 update. Thread t keeps row t/8,
+// columns 4(t%8)..+3 in registers and publishes them each step to one of two
+// LDS buffers, so one barrier per step orders the exchange; every LDS read of
+// a step is issued right after that barrier, and the pivot rows are handled
+// by the same update formula as the other rows (no divergent branch).
 template <typename T>
 __device__ void tile_inverse_diag(Lds<T> &L, int *status) {
   static_assert(TB * TB == 4 * NT, "4 elements per thread");
@@ -414,41 +438,65 @@ __device__ void tile_inverse_diag(Lds<T> &L, int *status) {
   for (int q = 0; q < 4; ++q) { ar[q] = L.Ar[i][c0 + q]; ai[q] = L.Ai[i][c0 + q]; }
   bool bad = false;
 #pragma unroll 2
-  for (int j = 0; j < TB; ++j) {
-    T (*Br)[TBP] = (j & 1) ? L.Rr : L.Pr;
-    T (*Bi)[TBP] = (j & 1) ? L.Ri : L.Pi;
+  for (int j = 0; j < TB; j += 2) {
+    T (*Br)[TBP] = (j & 2) ? L.Rr : L.Pr;
+    T (*Bi)[TBP] = (j & 2) ? L.Ri : L.Pi;
 #pragma unroll
     for (int q = 0; q < 4; ++q) { Br[i][c0 + q] = ar[q]; Bi[i][c0 + q] = ai[q]; }
     __syncthreads();
-    const T pr = Br[j][j], pim = Bi[j][j];
-    if (!(fabs(pr) + fabs(pim) > T(0)) || !isfinite(pr) || !isfinite(pim)) bad = true;
-    T dr, di;
-    fast_crecip<T>(pr, pim, dr, di);
-    if (i == j) {
+    // pivot block, this row's pivot columns, the pivot rows at these columns
+    const T p00r = Br[j][j], p00i = Bi[j][j], p01r = Br[j][j + 1], p01i = Bi[j][j + 1];
+    const T p10r = Br[j + 1][j], p10i = Bi[j + 1][j], p11r = Br[j + 1][j + 1], p11i = Bi[j + 1][j + 1];
+    const T f0r = Br[i][j], f0i = Bi[i][j], f1r = Br[i][j + 1], f1i = Bi[i][j + 1];
+    T h0r[4], h0i[4], h1r[4], h1i[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (c0 + q == j) { ar[q] = dr; ai[q] = di; }
-        else {
-          const T xr = ar[q], xi = ai[q];
-          ar[q] = xr * dr - xi * di;
-          ai[q] = xr * di + xi * dr;
-        }
-      }
-    } else {
-      const T fr = Br[i][j], fi = Bi[i][j];
-      const T gr = fr * dr - fi * di, gi = fr * di + fi * dr;  // f * d
+    for (int q = 0; q < 4; ++q) {
+      h0r[q] = Br[j][c0 + q]; h0i[q] = Bi[j][c0 + q];
+      h1r[q] = Br[j + 1][c0 + q]; h1i[q] = Bi[j + 1][c0 + q];
+    }
+    // Q = P^-1 = adj(P') sc / det(P') with P' = sc P, sc = 1 / max |component|
+    // (det P itself would overflow float for Hz, whose entries reach ~1e20)
+    const T mx = fmax(fmax(fmax(fabs(p00r), fabs(p00i)), fmax(fabs(p01r), fabs(p01i))),
+                      fmax(fmax(fabs(p10r), fabs(p10i)), fmax(fabs(p11r), fabs(p11i))));
+    const T sc = rcp_nr(mx);
+    const T a00r = p00r * sc, a00i = p00i * sc, a01r = p01r * sc, a01i = p01i * sc;
+    const T a10r = p10r * sc, a10i = p10i * sc, a11r = p11r * sc, a11i = p11i * sc;
+    const T dr = (a00r * a11r - a00i * a11i) - (a01r * a10r - a01i * a10i);
+    const T di = (a00r * a11i + a00i * a11r) - (a01r * a10i + a01i * a10r);
+    if (!(fabs(dr) + fabs(di) > T(0)) || !isfinite(dr) || !isfinite(di) || !isfinite(sc)) bad = true;
+    T er, ei;
+    fast_crecip<T>(dr, di, er, ei);
+    er *= sc;
+    ei *= sc;
+    const T q00r = a11r * er - a11i * ei, q00i = a11r * ei + a11i * er;
+    const T q01r = -(a01r * er - a01i * ei), q01i = -(a01r * ei + a01i * er);
+    const T q10r = -(a10r * er - a10i * ei), q10i = -(a10r * ei + a10i * er);
+    const T q11r = a00r * er - a00i * ei, q11i = a00r * ei + a00i * er;
+    // new row = base - g0 h0 - g1 h1: off the pivot rows base = row and
+    // g = [f0 f1] Q; on them base = 0 and g = -(row of Q), so the pivot rows
+    // become Q [h0; h1] without cancellation
+    T g0r = f0r * q00r - f0i * q00i + f1r * q10r - f1i * q10i;
+    T g0i = f0r * q00i + f0i * q00r + f1r * q10i + f1i * q10r;
+    T g1r = f0r * q01r - f0i * q01i + f1r * q11r - f1i * q11i;
+    T g1i = f0r * q01i + f0i * q01r + f1r * q11i + f1i * q11r;
+    const bool r0 = i == j, r1 = i == j + 1, prow = r0 || r1;
+    g0r = r0 ? -q00r : (r1 ? -q10r : g0r);
+    g0i = r0 ? -q00i : (r1 ? -q10i : g0i);
+    g1r = r0 ? -q01r : (r1 ? -q11r : g1r);
+    g1i = r0 ? -q01i : (r1 ? -q11i : g1i);
+    // the pivot columns: Q on the pivot rows, -g elsewhere
+    const T cjr = r0 ? q00r : (r1 ? q10r : -g0r), cji = r0 ? q00i : (r1 ? q10i : -g0i);
+    const T ckr = r0 ? q01r : (r1 ? q11r : -g1r), cki = r0 ? q01i : (r1 ? q11i : -g1i);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (c0 + q == j) { ar[q] = -gr; ai[q] = -gi; }
-        else {
-          const T hr = Br[j][c0 + q], hi = Bi[j][c0 + q];
-          ar[q] -= gr * hr - gi * hi;
-          ai[q] -= gr * hi + gi * hr;
-        }
-      }
+    for (int q = 0; q < 4; ++q) {
+      const T br = prow ? T(0) : ar[q], bi = prow ? T(0) : ai[q];
+      const T xr = br - (g0r * h0r[q] - g0i * h0i[q]) - (g1r * h1r[q] - g1i * h1i[q]);
+      const T xi = bi - (g0r * h0i[q] + g0i * h0r[q]) - (g1r * h1i[q] + g1i * h1r[q]);
+      const int c = c0 + q;
+      ar[q] = c == j ? cjr : (c == j + 1 ? ckr : xr);
+      ai[q] = c == j ? cji : (c == j + 1 ? cki : xi);
     }
   }
-  // (Ar, Ai) is not read inside the loop
 #pragma unroll
   for (int q = 0; q < 4; ++q) { L.Ar[i][c0 + q] = ar[q]; L.Ai[i][c0 + q] = ai[q]; }
   if (bad) *status = 1;
@@ -598,10 +646,16 @@ __device__ inline void schur_next_00(const FJob<T> &J, int k, Lds<T> &L) {
 template <typename T, bool PIV>
 __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
   __shared__ Lds<T> L;
-  int jb = 0;
-  while (jb + 1 < batch.njobs && (int)blockIdx.x >= batch.wg_start[jb + 1]) ++jb;
+  int jb = 0, local = 0;
+  if ((int)blockIdx.x < batch.njobs) {
+    jb = blockIdx.x;
+    local = critical_tile(batch.job[jb]);
+  } else {
+    while (jb + 1 < batch.njobs && (int)blockIdx.x >= batch.wg_start[jb + 1]) ++jb;
+    const int r = blockIdx.x - batch.wg_start[jb], c = critical_tile(batch.job[jb]);
+    local = r < c ? r : r + 1;
+  }
   const FJob<T> &J = batch.job[jb];
-  const int local = blockIdx.x - batch.wg_start[jb];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int qi = w >> 1, qj = w & 1;
   const size_t np2 = (size_t)J.np * J.np;
@@ -1608,8 +1662,9 @@ inline double *wrap_kinv(const fdfd_grid *g, int precision, const void *factor) 
 
 template <typename T, bool PIV>
 int launch_step(const FBatch<T> &b, hipStream_t s) {
-  const int grid = b.wg_start[b.njobs];
-  hipLaunchKernelGGL((k_gj_step<T, PIV>), dim3(grid), dim3(NT), 0, s, b);
+  FBatch<T> bl = b;
+  const int grid = batch_layout(bl);
+  hipLaunchKernelGGL((k_gj_step<T, PIV>), dim3(grid), dim3(NT), 0, s, bl);
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
 }
@@ -1658,10 +1713,8 @@ int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, 
   J.op = op; J.G = (T *)factor; J.W = (T *)fw.W; J.Pinv = (T *)fw.Pinv; J.status = fw.status;
   J.m = m; J.n = n; J.np = np; J.nt = nt;
   J.k = -1; J.p = 0;
-  b.wg_start[0] = 0; b.wg_start[1] = 1;
   int rc = launch_step<T, PIV>(b, s);
   if (rc) return rc;
-  b.wg_start[1] = nt * nt;
   for (int k = 0; k < m; ++k) {
     for (int p = 0; p < nt; ++p) {
       J.k = k; J.p = p;
@@ -1855,13 +1908,8 @@ int engine_launch(fdfd_engine *E, std::vector<std::shared_ptr<EngineJob>> &sel) 
   FBatch<T> b;
   memset(&b, 0, sizeof b);
   b.njobs = (int)sel.size();
-  int wg = 0;
-  for (int j = 0; j < b.njobs; ++j) {
-    fill_job<T>(b.job[j], *sel[j]);
-    b.wg_start[j] = wg;
-    wg += sel[j]->k < 0 ? 1 : sel[j]->nt * sel[j]->nt;
-  }
-  b.wg_start[b.njobs] = wg;
+  for (int j = 0; j < b.njobs; ++j) fill_job<T>(b.job[j], *sel[j]);
+  const int wg = batch_layout(b);
   hipLaunchKernelGGL((k_gj_step<T, PIV>), dim3(wg), dim3(NT), 0, E->stream, b);
   hipError_t err = hipGetLastError();
   for (auto &e : sel) {

@@ -177,7 +177,7 @@ int fdfd_fields(const fdfd_grid *g, const double *eps, const double *x, double *
 /* Step-level batching engine (no reference counterpart; serves the request
  * loop). A scheduler thread with its own stream advances every submitted
  * factorization by one Gauss-Jordan step per launch; the factorizations in
- * flight (up to 8 per launch, grouped by precision) share each launch and
+ * flight (up to 16 per launch, grouped by precision) share each launch and
  * new ones join at the next launch. Arguments of fdfd_engine_submit are as
  * for fdfd_factor and must stay valid until fdfd_engine_wait returns; the
  * operator must be complete on the device before submission.
