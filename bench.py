@@ -30,6 +30,11 @@ sys.path.insert(0, ROOT)
 
 FP32_MFMA_PEAK_TFLOPS = 157.3
 FP64_MFMA_PEAK_TFLOPS = 78.6
+# HBM bytes per k_gj_step<float, false> launch at n = 512 (FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000, calibrated on a 256 MiB copy), from the rocprofv3 --pmc
+# runs of `bench.py --roofline-only` summarized in
+# profiles/k_gj_step_c64_512_roofline.md
+GJ_STEP_C64_512_TRAFFIC_BYTES = 5.44e6
 
 
 def log(*a):
@@ -92,7 +97,8 @@ def roofline_leg(n=512, precision="c64"):
     return {
         "kernel": f"k_gj_step<{tname}, false>",
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-        "frac": round(achieved / peak, 5), "traffic": None,
+        "frac": round(achieved / peak, 5),
+        "traffic": GJ_STEP_C64_512_TRAFFIC_BYTES if (n, precision) == (512, "c64") else None,
         "workload": f"complex factor of a {n}x{n} Ez operator: {launches} launches, "
                     f"8*n^4 = {flops:.3e} flops per factor, 8*n^2*32 = {8.0 * n * n * 32:.3e} per launch",
         "avg_launch_us": round(ms * 1e3 / launches, 3), "factor_ms": round(ms, 3),
