@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--roofline-only", action="store_true", help="only the dominant-kernel measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a plain 256 MiB device copy (scale for the PMC byte counters)")
+    ap.add_argument("--configs", action="store_true",
+                    help="only the per-workload timings of configs 1, 2, 4 and 5 (one JSON object)")
     return ap.parse_args()
 
 
@@ -164,6 +166,78 @@ def cpu_baseline(reqs, seconds):
             "per_solve_s": {str(n): round(per_solve[n], 3) for n in sizes}}
 
 
+def _sync_time(fn):
+    import torch
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t
+
+
+def config_timings():
+    """Wall times of the other workloads on one GPU (second of two runs each)."""
+    import numpy as np
+    import torch
+    from fdfdpy_amd import configs as C
+    from fdfdpy_amd.adjoint import FieldSolver
+    from fdfdpy_amd.device import DeviceOperator
+    from fdfdpy_amd.simulation import Simulation
+    out = {}
+    # 1: Ez 120x120, one point source, FP64 factor and solve through Simulation
+    p = C.c1_point()
+    for _ in range(2):
+        sim = Simulation(p.omega, p.eps, p.dl, p.npml, 'Ez')
+        sim.src = p.srcs[0]
+        (_, _, ez), t = _sync_time(lambda: sim.solve_fields())
+    out["c1_point_Ez_120"] = {"solve_fields_s": round(t, 4), "note": "assembly done at construction; "
+                              "factor + refinement + secondary fields"}
+    # 2: Hz 512x512, 8 point sources: c64 factor + FP64 refinement against the FP64 factor
+    p = C.c2_design()
+    b = torch.as_tensor(p.rhs(), device="cuda:0")
+    res = {}
+    for prec in ("c64", "fp64"):
+        for _ in range(2):
+            op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, "cuda:0")
+            op.set_eps(p.eps)
+            _, tf = _sync_time(lambda: op.factor(prec))
+            (x, rep_), ts = _sync_time(lambda: op.solve(b, precision=prec, tol=1e-10))
+        res[prec] = (x, rep_, tf, ts)
+    xm, xf = res["c64"][0], res["fp64"][0]
+    err = [float(torch.linalg.vector_norm(xm[q] - xf[q]) / torch.linalg.vector_norm(xf[q])) for q in range(8)]
+    out["c2_design_Hz_512_8rhs"] = {
+        prec: {"factor_s": round(r[2], 4), "solve_8rhs_s": round(r[3], 4), "steps": r[1].iterations,
+               "max_rel_residual": r[1].max_rel_residual} for prec, r in res.items()}
+    out["c2_design_Hz_512_8rhs"]["mixed_vs_fp64_max_rel_field_diff"] = max(err)
+    # 4: Ez 300x300 adjoint gradient over the whole grid (forward + adjoint solve, resident factor)
+    p = C.c4_adjoint()
+    probe = torch.tensor(p.extra["probe"], device="cuda:0")
+    src = torch.tensor(p.srcs[0], device="cuda:0")
+    for _ in range(2):
+        solver = FieldSolver(p.shape, p.omega, p.dl, p.npml, device="cuda:0")
+        eps = torch.tensor(p.eps, device="cuda:0", requires_grad=True)
+
+        def grad():
+            Ez = solver(eps, src)
+            (Ez.abs() ** 2 * probe).sum().backward()
+            return eps.grad
+        _, t = _sync_time(grad)
+    out["c4_adjoint_Ez_300"] = {"objective_and_gradient_s": round(t, 4),
+                                "note": "assembly, factor, forward solve, adjoint solve, gradient kernel"}
+    # 5: Ez 400x400 Kerr Born iteration, 8 amplitudes, to relative change 1e-10
+    p = C.c5_kerr()
+    per_amp = []
+    for a in sorted(p.extra["amps"]):
+        sim = Simulation(p.omega, p.eps, p.dl, p.npml, "Ez")
+        sim.add_nl(p.extra["chi3"], p.extra["nl_region"])
+        sim.src = p.srcs[0] * a
+        (_, _, _, conv), t = _sync_time(lambda: sim.solve_fields_nl(solver_nl="born", conv_threshold=1e-10,
+                                                                      max_num_iter=50))
+        per_amp.append({"amp": round(float(a), 3), "born_steps": int(np.count_nonzero(conv)), "s": round(t, 4)})
+    out["c5_kerr_Ez_400"] = per_amp
+    return out
+
+
 def run_stream(reqs, precision, slots, tol):
     import torch
     from fdfdpy_amd.serving import ServingEngine, summarize
@@ -197,6 +271,9 @@ def main():
         return
     if args.copy_calibration:
         print(json.dumps(copy_calibration()))
+        return
+    if args.configs:
+        print(json.dumps(config_timings()))
         return
 
     n_req = args.steps * args.batch
