@@ -680,21 +680,57 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
   T *Pnext = J.Pinv + (size_t)((step + 1) & 1) * 2 * TB * TB;
 
   // every global read of this tile's update, issued together
-  for (int e = tid; e < TB * TB; e += NT) {
-    L.Pr[e / TB][e % TB] = Pslot[e];
-    L.Pi[e / TB][e % TB] = Pslot[TB * TB + e];
-  }
-  if (tj != p) load_tile(J, k, p, src, p, tj, L.Ar, L.Ai);   // X[p, J]
-  if (ti != p) load_tile(J, k, p, src, ti, p, L.Br, L.Bi);   // X[I, p]
   typename Mfma<T>::v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-  if (ti != p && tj != p) {
+  if (p > 0) {
+    // plain loads of the previous launch's output: all of them before any use
+    constexpr int PER = TB * TB / NT;
+    T pr[PER], pim[PER], ar[PER], ai[PER], br[PER], bi[PER];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
-      T xr, xi;
-      x_elem(J, k, p, src, ti * TB + a, tj * TB + b, xr, xi);
-      cr[t] = xr;
-      ci[t] = xi;
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + u * NT, a = e / TB, b = e % TB;
+      pr[u] = Pslot[e];
+      pim[u] = Pslot[TB * TB + e];
+      const size_t ia = (size_t)(p * TB + a) * J.np + tj * TB + b;
+      const size_t ib = (size_t)(ti * TB + a) * J.np + p * TB + b;
+      ar[u] = tj != p ? src[ia] : T(0);
+      ai[u] = tj != p ? src[np2 + ia] : T(0);
+      br[u] = ti != p ? src[ib] : T(0);
+      bi[u] = ti != p ? src[np2 + ib] : T(0);
+    }
+    if (ti != p && tj != p) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+        const size_t idx = (size_t)(ti * TB + a) * J.np + tj * TB + b;
+        cr[t] = src[idx];
+        ci[t] = src[np2 + idx];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + u * NT, a = e / TB, b = e % TB;
+      L.Pr[a][b] = pr[u];
+      L.Pi[a][b] = pim[u];
+      if (tj != p) { L.Ar[a][b] = ar[u]; L.Ai[a][b] = ai[u]; }
+      if (ti != p) { L.Br[a][b] = br[u]; L.Bi[a][b] = bi[u]; }
+    }
+  } else {
+    // p == 0: the tiles of S_k = T_k - l_k G_{k-1} u_{k-1}, formed on load
+    for (int e = tid; e < TB * TB; e += NT) {
+      L.Pr[e / TB][e % TB] = Pslot[e];
+      L.Pi[e / TB][e % TB] = Pslot[TB * TB + e];
+    }
+    if (tj != p) load_tile(J, k, p, src, p, tj, L.Ar, L.Ai);   // X[p, J]
+    if (ti != p) load_tile(J, k, p, src, ti, p, L.Br, L.Bi);   // X[I, p]
+    if (ti != p && tj != p) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
+        T xr, xi;
+        x_elem(J, k, p, src, ti * TB + a, tj * TB + b, xr, xi);
+        cr[t] = xr;
+        ci[t] = xi;
+      }
     }
   }
   __syncthreads();
@@ -780,12 +816,36 @@ __device__ inline void publish_tile(T *dst, const typename Mfma<T>::v4 &cr, cons
     st_xcd(dst + TB * TB + e, ci[t]);
   }
 }
+// Published tiles are read with every load of the call issued before the
+// first LDS store, so their latencies overlap instead of adding up.
+template <typename T, int NTILE>
+__device__ inline void fetch_tiles(const T *const (&src)[NTILE], T (*const (&dr)[NTILE])[TBP],
+                                   T (*const (&di)[NTILE])[TBP]) {
+  constexpr int PER = TB * TB / NT;
+  T vr[NTILE][PER], vi[NTILE][PER];
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * NT;
+      vr[t][u] = ld_xcd(src[t] + e);
+      vi[t][u] = ld_xcd(src[t] + TB * TB + e);
+    }
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * NT;
+      dr[t][e / TB][e % TB] = vr[t][u];
+      di[t][e / TB][e % TB] = vi[t][u];
+    }
+}
 template <typename T>
 __device__ inline void fetch_tile(const T *src, T (*dr)[TBP], T (*di)[TBP]) {
-  for (int e = threadIdx.x; e < TB * TB; e += NT) {
-    dr[e / TB][e % TB] = ld_xcd(src + e);
-    di[e / TB][e % TB] = ld_xcd(src + TB * TB + e);
-  }
+  const T *const s1[1] = {src};
+  T (*const d1r[1])[TBP] = {dr};
+  T (*const d1i[1])[TBP] = {di};
+  fetch_tiles<T, 1>(s1, d1r, d1i);
 }
 template <typename T>
 __device__ inline void acc_to_lds(const typename Mfma<T>::v4 &cr, const typename Mfma<T>::v4 &ci,
@@ -894,9 +954,12 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
         wait_flag(fpinv + p, tag, abort, J.cap);
         wait_flag(frow + p * nt + tj, tag, abort, J.cap);
         wait_flag(fcol + p * nt + ti, tag, abort, J.cap);
-        fetch_tile<T>(ppiv, L.Pr, L.Pi);
-        fetch_tile<T>(prow, L.Ar, L.Ai);
-        fetch_tile<T>(xcol + ((size_t)p * nt + ti) * tsz, L.Br, L.Bi);
+        {
+          const T *const s3[3] = {ppiv, prow, xcol + ((size_t)p * nt + ti) * tsz};
+          T (*const d3r[3])[TBP] = {L.Pr, L.Ar, L.Br};
+          T (*const d3i[3])[TBP] = {L.Pi, L.Ai, L.Bi};
+          fetch_tiles<T, 3>(s3, d3r, d3i);
+        }
         __syncthreads();
         v4 rr = {0, 0, 0, 0}, ri = {0, 0, 0, 0};
         cgemm_quad<T>(rr, ri, L.Pr, L.Pi, L.Ar, L.Ai, qi, qj, lane, T(1));
@@ -1171,17 +1234,33 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
     if (s > 0) wait_all_flags(sy->done[dir], W, (unsigned)s, &sy->abort, cap);
     else __syncthreads();
     if (coupled) {
+      // every load of the previous block's vector is issued before any use
       const int kp = dir == 0 ? k - 1 : k + 1;
-      for (int e = tid; e < nrhs * nv; e += NT) {
-        const int q = e / nv, c = e - q * nv;
-        if (c < ny) {
+      constexpr int CU = (768 + NT - 1) / NT;   // columns per thread per RHS
+      T yr[RP][CU], yi[RP][CU];
+#pragma unroll
+      for (int q = 0; q < RP; ++q)
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+          const int c = tid + u * NT;
+          const bool ok = q < nrhs && c < ny;
           const T *yp = y + 2 * (((size_t)q * nx + kp) * np + c);
-          const double yr = (double)ld_xcd(yp), yi = (double)ld_xcd(yp + 1);
-          const double tr = cpr[c] * yr - cpi[c] * yi, ti = cpr[c] * yi + cpi[c] * yr;
-          if (dir == 0) { vr[e] = (T)((double)vr[e] - tr); vi[e] = (T)((double)vi[e] - ti); }
-          else { vr[e] = (T)tr; vi[e] = (T)ti; }
+          yr[q][u] = ok ? ld_xcd(yp) : T(0);
+          yi[q][u] = ok ? ld_xcd(yp + 1) : T(0);
         }
-      }
+#pragma unroll
+      for (int q = 0; q < RP; ++q)
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+          const int c = tid + u * NT;
+          if (q < nrhs && c < ny) {
+            const int e = q * nv + c;
+            const double ar = (double)yr[q][u], ai = (double)yi[q][u];
+            const double tr = cpr[c] * ar - cpi[c] * ai, ti = cpr[c] * ai + cpi[c] * ar;
+            if (dir == 0) { vr[e] = (T)((double)vr[e] - tr); vi[e] = (T)((double)vi[e] - ti); }
+            else { vr[e] = (T)tr; vi[e] = (T)ti; }
+          }
+        }
     }
     __syncthreads();
     // 4. partial row dot products for every (row, rhs) pair of this wave
