@@ -1220,15 +1220,30 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
         cpi[c] = cpl.i;
       }
     }
-    for (int e = tid; e < RP * nv; e += NT) {
-      const int q = e / nv, c = e - q * nv;
-      T pr = 0, pi = 0;
-      if (dir == 0 && q < nrhs && c < ny) {
-        cd v = ld_c(r, (size_t)q * M + (size_t)k * ny + c);
-        pr = (T)v.r; pi = (T)v.i;
-      }
-      vr[e] = pr;
-      vi[e] = pi;
+    {
+      // the residual of block k (forward) or zeros, loads issued together
+      constexpr int CU = (768 + NT - 1) / NT;
+      double rr[RP][CU], ri[RP][CU];
+#pragma unroll
+      for (int q = 0; q < RP; ++q)
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+          const int c = tid + u * NT;
+          const bool ok = dir == 0 && q < nrhs && c < ny;
+          const double *rp = r + 2 * ((size_t)q * M + (size_t)k * ny + c);
+          rr[q][u] = ok ? rp[0] : 0.0;
+          ri[q][u] = ok ? rp[1] : 0.0;
+        }
+#pragma unroll
+      for (int q = 0; q < RP; ++q)
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+          const int c = tid + u * NT;
+          if (c < nv) {
+            vr[q * nv + c] = (T)rr[q][u];
+            vi[q * nv + c] = (T)ri[q][u];
+          }
+        }
     }
     // 3. wait for the previous block, then fold it into v
     if (s > 0) wait_all_flags(sy->done[dir], W, (unsigned)s, &sy->abort, cap);
@@ -1426,66 +1441,60 @@ __global__ void __launch_bounds__(NT) k_wrap_capacitance(fdfd_grid g, const doub
   st_c(K, (size_t)i * N + j, v);
 }
 
-// Gauss-Jordan inverse of K in place with partial pivoting: per column j one
-// single-workgroup pivot launch (row swap, column copy, row scaling) and one
-// update launch over the whole matrix; the column swaps are undone last.
-__global__ void __launch_bounds__(NT) k_kgj_pivot(double *K, int N, int j, double *colj, int *piv,
-                                                  int *status) {
+// Gauss-Jordan inverse of K with partial pivoting, one launch per column j:
+// the launch reads the matrix before step j from one buffer and writes the
+// matrix after it to the other, so no element is read after it is written.
+// Workgroup (bx, by) owns rows [16 by, 16 by + 16) x columns [256 bx, +256)
+// and repeats the pivot search over column j (identical result everywhere).
+// Row interchanges are applied as the rows are written; the column swaps are
+// undone at the end (k_kgj_unpermute).
+constexpr int KGJ_ROWS = 16;
+__global__ void __launch_bounds__(NT) k_kgj_col(const double *__restrict__ Ks, double *__restrict__ Kd, int N,
+                                                int j, int *piv, int *status) {
   __shared__ double bv[NT];
   __shared__ int bi[NT];
   const int tid = threadIdx.x;
   double best = -1;
   int arg = j;
   for (int i = j + tid; i < N; i += NT) {
-    const cd z = ld_c(K, (size_t)i * N + j);
+    const cd z = ld_c(Ks, (size_t)i * N + j);
     const double v = fabs(z.r) + fabs(z.i);
     if (v > best) { best = v; arg = i; }
   }
   bv[tid] = best;
   bi[tid] = arg;
   __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (tid < s) {
-      const double o = bv[tid + s];
-      const int oi = bi[tid + s];
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      const double o = bv[tid + st];
+      const int oi = bi[tid + st];
       if (o > bv[tid] || (o == bv[tid] && oi < bi[tid])) { bv[tid] = o; bi[tid] = oi; }
     }
     __syncthreads();
   }
   const int p = bi[0];
-  if (tid == 0) {
+  const cd pv = ld_c(Ks, (size_t)p * N + j);
+  const cd d = (pv.r == 0 && pv.i == 0) ? cd{0, 0} : cinv(pv);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
     piv[j] = p;
     if (!(bv[0] > 0) || !isfinite(bv[0])) status[0] = 1;
   }
-  if (p != j) {
-    for (int c = tid; c < N; c += NT) {
-      const cd x = ld_c(K, (size_t)j * N + c), y = ld_c(K, (size_t)p * N + c);
-      st_c(K, (size_t)j * N + c, y);
-      st_c(K, (size_t)p * N + c, x);
+  const int c = blockIdx.x * NT + tid;
+  if (c >= N) return;
+  const cd h = ld_c(Ks, (size_t)p * N + c);   // pivot row (row p before the interchange)
+  for (int r = 0; r < KGJ_ROWS; ++r) {
+    const int i = blockIdx.y * KGJ_ROWS + r;
+    if (i >= N) break;
+    cd v;
+    if (i == j) {
+      v = c == j ? d : cmul(h, d);
+    } else {
+      const int sr = i == p ? j : i;   // row p now holds the old row j
+      const cd f = ld_c(Ks, (size_t)sr * N + j);
+      const cd g = cmul(f, d);
+      v = c == j ? cd{-g.r, -g.i} : csub(ld_c(Ks, (size_t)sr * N + c), cmul(g, h));
     }
-  }
-  __syncthreads();
-  for (int i = tid; i < N; i += NT) st_c(colj, i, ld_c(K, (size_t)i * N + j));
-  const cd pv = ld_c(K, (size_t)j * N + j);
-  const cd d = (pv.r == 0 && pv.i == 0) ? cd{0, 0} : cinv(pv);
-  __syncthreads();
-  for (int c = tid; c < N; c += NT) {
-    const size_t e = (size_t)j * N + c;
-    st_c(K, e, c == j ? d : cmul(ld_c(K, e), d));
-  }
-}
-
-__global__ void __launch_bounds__(NT) k_kgj_update(double *K, int N, int j, const double *__restrict__ colj) {
-  const int c = blockIdx.x * NT + threadIdx.x, i = blockIdx.y;
-  if (c >= N || i == j) return;
-  const cd f = ld_c(colj, i);
-  const size_t e = (size_t)i * N + c;
-  if (c == j) {
-    const cd d = ld_c(K, (size_t)j * N + j);
-    const cd v = cmul(f, d);
-    st_c(K, e, cd{-v.r, -v.i});
-  } else {
-    st_c(K, e, csub(ld_c(K, e), cmul(f, ld_c(K, (size_t)j * N + c))));
+    st_c(Kd, (size_t)i * N + c, v);
   }
 }
 
@@ -1679,6 +1688,7 @@ inline size_t elem_size(int precision) { return base_prec(precision) == FDFD_PRE
 struct FactorWork {
   void *W; void *Pinv; int *status;
   void *xcol; void *pinv_all; unsigned *sync;  // persistent-kernel exchange (W doubles as xrow)
+  double *kscratch;                            // ping-pong partner of K (order 2n, complex FP64)
 };
 size_t persist_sync_words(int nt) { return 2 + (size_t)nt + 2 * (size_t)nt * nt; }
 FactorWork factor_work_layout(const fdfd_grid *g, int precision, void *work) {
@@ -1691,13 +1701,15 @@ FactorWork factor_work_layout(const fdfd_grid *g, int precision, void *work) {
   fw.W = base + off;        off += align_up(2 * 2 * np * np * es, 256);  // also xrow (2 parities)
   fw.xcol = base + off;     off += align_up(2 * 2 * np * np * es, 256);
   fw.pinv_all = base + off; off += align_up(2 * nt * 2 * TB * TB * es, 256);
-  fw.sync = (unsigned *)(base + off);
+  fw.sync = (unsigned *)(base + off); off += align_up(4 * persist_sync_words((int)nt), 256);
+  fw.kscratch = (double *)(base + off);
   return fw;
 }
 size_t factor_work_bytes(const fdfd_grid *g, int precision) {
-  const size_t np = pad_n(g->ny), nt = np / TB, es = elem_size(precision);
+  const size_t np = pad_n(g->ny), nt = np / TB, es = elem_size(precision), N = 2 * (size_t)g->ny;
   return 256 + align_up(2 * 2 * TB * TB * es, 256) + 2 * align_up(2 * 2 * np * np * es, 256) +
-         align_up(2 * nt * 2 * TB * TB * es, 256) + align_up(4 * persist_sync_words((int)nt), 256);
+         align_up(2 * nt * 2 * TB * TB * es, 256) + align_up(4 * persist_sync_words((int)nt), 256) +
+         align_up(16 * N * N, 256);
 }
 
 struct SolveWork {
@@ -1865,13 +1877,15 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, h
   double *K = wrap_kinv(g, prec, factor);
   hipLaunchKernelGGL(k_wrap_capacitance<T>, dim3((N + NT - 1) / NT, N), dim3(NT), 0, s, *g, op, Glast, sig_prev,
                      z_prev, np, K);
-  // Gauss-Jordan scratch in the (now free) pivot-inverse area
-  double *colj = (double *)fw.pinv_all;
-  int *piv = (int *)(colj + 2 * (size_t)N);
-  for (int j = 0; j < N; ++j) {
-    hipLaunchKernelGGL(k_kgj_pivot, dim3(1), dim3(NT), 0, s, K, N, j, colj, piv, fw.status);
-    hipLaunchKernelGGL(k_kgj_update, dim3((N + NT - 1) / NT, N), dim3(NT), 0, s, K, N, j, (const double *)colj);
-  }
+  // Gauss-Jordan: the pivot record in the (now free) pivot-inverse area,
+  // the ping-pong partner of K in the workspace's K scratch
+  int *piv = (int *)fw.pinv_all;
+  double *Kb[2] = {K, fw.kscratch};
+  const dim3 kgrid((N + NT - 1) / NT, (N + KGJ_ROWS - 1) / KGJ_ROWS);
+  for (int j = 0; j < N; ++j)
+    hipLaunchKernelGGL(k_kgj_col, kgrid, dim3(NT), 0, s, (const double *)Kb[j & 1], Kb[(j + 1) & 1], N, j, piv,
+                       fw.status);
+  if (N & 1) HIP_TRY(hipMemcpyAsync(K, fw.kscratch, 16 * (size_t)N * N, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_kgj_unpermute, dim3((N + NT - 1) / NT), dim3(NT), 0, s, K, N, (const int *)piv);
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
