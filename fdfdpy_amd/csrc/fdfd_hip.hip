@@ -1110,14 +1110,20 @@ __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const
 // [w*4*SRW, (w+1)*4*SRW) of every block; lane l of a wave covers columns
 // l, l+64, ... of its rows. Blocks are chained through per-workgroup flags:
 // step s waits until all W workgroups have finished step s-1. Each workgroup
-// loads its rows of G_k, the coupling coefficients and (backward) its own
-// entries of y_k before it waits, so HBM latency overlaps the
-// synchronization. A wait that exceeds SPIN_CAP polls raises the abort flag;
-// every later wait then returns at once, so the grid always drains.
-// The vectors passed between workgroups (y_k, d_k) are written and read with
-// agent-scope relaxed atomics, which are coherent across the XCDs' L2 caches
-// by themselves; the flags need only workgroup-scope ordering around them,
-// so no step writes back or invalidates a whole L2. The row sums of a wave
+// computes its rows of y_k (forward) or d_k (backward) and, from them, its
+// rows of the next block's input vector (v_{k+1} = r_{k+1} - XM_{k+1} y_k
+// forward, w_{k-1} = XP_{k-1} d_k backward), which it publishes; the next
+// step then only loads that vector, with no per-workgroup recomputation.
+// Rows of G_k and the inputs of the published entries are loaded before the
+// wait, so HBM latency overlaps the synchronization. A wait that exceeds
+// SPIN_CAP polls raises the abort flag; every later wait then returns at
+// once, so the grid always drains.
+// The published vectors (two buffers alternating by step parity) are written
+// and read with agent-scope relaxed atomics, which are coherent across the
+// XCDs' L2 caches by themselves; the flags need only workgroup-scope ordering
+// around them, so no step writes back or invalidates a whole L2. A buffer is
+// rewritten two steps after it was read, and every reader has passed a flag
+// since. The row sums of a wave
 // (SRW rows x RP right-hand sides x re/im) are reduced together. RP, the
 // right-hand sides of one pass rounded up to a power of two, is a template
 // parameter so the unrolled loops stay small enough for the instruction
@@ -1137,6 +1143,8 @@ struct SweepSync {
   unsigned abort;
   unsigned pad[63];
   unsigned done[2][MAXW];   // forward / backward: steps finished per workgroup
+  // the next block's input vector, published row by row: [2 parity][RP][np] complex
+  double vbuf[2 * MAXRHS * 768];
 };
 
 // Sums v[0..N) over the 64 lanes of a wave at once: each round halves the
@@ -1167,17 +1175,27 @@ __device__ inline T wave_multi_sum(T (&v)[N], int lane) {
   return t;
 }
 
+// a complex entry of a published vector, one load per entry
+__device__ inline void ld_xcd_c(const float *p, float &re, float &im) {
+  const unsigned long long v = ld_xcd(reinterpret_cast<const unsigned long long *>(p));
+  re = __uint_as_float((unsigned)(v & 0xffffffffull));
+  im = __uint_as_float((unsigned)(v >> 32));
+}
+__device__ inline void ld_xcd_c(const double *p, double &re, double &im) {
+  re = ld_xcd(p);
+  im = ld_xcd(p + 1);
+}
+
 template <typename T, int RP>
 __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const double *__restrict__ op,
                                                const double *__restrict__ r, T *y, double *__restrict__ x,
                                                int nx, int ny, int np, int nrhs, int dir, SweepSync *sy,
                                                unsigned cap) {
   extern __shared__ unsigned char smem_raw[];
-  double *cpr = reinterpret_cast<double *>(smem_raw);   // [np] coupling to the previous block
-  double *cpi = cpr + np;
   const int nv = (np + 63) / 64 * 64;                  // padded row length of v
-  T *vr = reinterpret_cast<T *>(cpi + np);             // [RP][nv]
+  T *vr = reinterpret_cast<T *>(smem_raw);             // [RP][nv]
   T *vi = vr + (size_t)RP * nv;
+  T *vb = reinterpret_cast<T *>(sy->vbuf);             // [2][RP][np][2]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const size_t np2 = (size_t)np * np, M = (size_t)nx * ny;
   const int W = gridDim.x;
@@ -1189,12 +1207,17 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
   const int my_ri = jv & 1, my_q = (jv >> 1) % RP, my_rr = jv / (2 * RP);
   const int my_a = row0 + my_rr;
   const bool my_store = (lane & ((1 << SH) - 1)) == 0 && my_q < nrhs && my_a < np;
+  const bool my_real = my_q < nrhs && my_a < ny;
+  const int cpl_plane = dir == 0 ? FDFD_OP_XM : FDFD_OP_XP;
   for (int s = 0; s < nx; ++s) {
     const int k = dir == 0 ? s : nx - 1 - s;
     const T *Gk = G + (size_t)k * 2 * np2;
     const bool coupled = dir == 0 ? k > 0 : k + 1 < nx;
     const bool active = dir == 0 || coupled;
-    // 1. this workgroup's rows of G_k (independent of the previous block)
+    const int kn = dir == 0 ? k + 1 : k - 1;             // the next block
+    const bool has_next = dir == 0 ? k + 1 < nx : k > 0;
+    // 1. this workgroup's rows of G_k and the inputs of its published
+    //    entries (independent of the previous block)
     T gr[SRW][MAXCL], gi[SRW][MAXCL];
 #pragma unroll
     for (int rr = 0; rr < SRW; ++rr) {
@@ -1207,23 +1230,20 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
         gi[rr][cc] = ok ? Gk[np2 + (size_t)a * np + c] : T(0);
       }
     }
-    // backward: this lane's own entry of y_k, written by the forward launch
-    T yown = 0;
+    cd rn = {0, 0}, cn = {0, 0};   // forward: r_{k+1}, XM_{k+1}; backward: XP_{k-1} (this lane's row)
+    if (my_real && has_next) {
+      if (dir == 0) rn = ld_c(r, (size_t)my_q * M + (size_t)kn * ny + my_a);
+      cn = ld_c(op, cpl_plane * M + (size_t)kn * ny + my_a);
+    }
+    T yown = 0;   // backward: this lane's entry of y_k, written by the forward launch
     T *my_y = y + 2 * (((size_t)my_q * nx + k) * np + my_a) + my_ri;
     if (dir == 1 && my_store) yown = *my_y;
-    // 2. the parts of v that do not depend on the previous block
-    if (coupled) {
-      const int pl = dir == 0 ? FDFD_OP_XM : FDFD_OP_XP;
-      for (int c = tid; c < ny; c += NT) {
-        const cd cpl = ld_c(op, pl * M + (size_t)k * ny + c);
-        cpr[c] = cpl.r;
-        cpi[c] = cpl.i;
-      }
-    }
-    {
-      // the residual of block k (forward) or zeros, loads issued together
+    // 2. this block's input vector into LDS: r_0 at the first forward step,
+    //    zero at the first backward step, else the vector all workgroups
+    //    published at the previous step (every load issued before any use)
+    if (s == 0) {
       constexpr int CU = (768 + NT - 1) / NT;
-      double rr[RP][CU], ri[RP][CU];
+      double ar[RP][CU], ai[RP][CU];
 #pragma unroll
       for (int q = 0; q < RP; ++q)
 #pragma unroll
@@ -1231,8 +1251,8 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
           const int c = tid + u * NT;
           const bool ok = dir == 0 && q < nrhs && c < ny;
           const double *rp = r + 2 * ((size_t)q * M + (size_t)k * ny + c);
-          rr[q][u] = ok ? rp[0] : 0.0;
-          ri[q][u] = ok ? rp[1] : 0.0;
+          ar[q][u] = ok ? rp[0] : 0.0;
+          ai[q][u] = ok ? rp[1] : 0.0;
         }
 #pragma unroll
       for (int q = 0; q < RP; ++q)
@@ -1240,45 +1260,38 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
         for (int u = 0; u < CU; ++u) {
           const int c = tid + u * NT;
           if (c < nv) {
-            vr[q * nv + c] = (T)rr[q][u];
-            vi[q * nv + c] = (T)ri[q][u];
+            vr[q * nv + c] = (T)ar[q][u];
+            vi[q * nv + c] = (T)ai[q][u];
           }
         }
-    }
-    // 3. wait for the previous block, then fold it into v
-    if (s > 0) wait_all_flags(sy->done[dir], W, (unsigned)s, &sy->abort, cap);
-    else __syncthreads();
-    if (coupled) {
-      // every load of the previous block's vector is issued before any use
-      const int kp = dir == 0 ? k - 1 : k + 1;
-      constexpr int CU = (768 + NT - 1) / NT;   // columns per thread per RHS
-      T yr[RP][CU], yi[RP][CU];
+      __syncthreads();
+    } else {
+      wait_all_flags(sy->done[dir], W, (unsigned)s, &sy->abort, cap);
+      const T *vin = vb + (size_t)(s & 1) * RP * np * 2;
+      constexpr int CU = (768 + NT - 1) / NT;
+      T br[RP][CU], bi[RP][CU];
 #pragma unroll
       for (int q = 0; q < RP; ++q)
 #pragma unroll
         for (int u = 0; u < CU; ++u) {
           const int c = tid + u * NT;
-          const bool ok = q < nrhs && c < ny;
-          const T *yp = y + 2 * (((size_t)q * nx + kp) * np + c);
-          yr[q][u] = ok ? ld_xcd(yp) : T(0);
-          yi[q][u] = ok ? ld_xcd(yp + 1) : T(0);
+          br[q][u] = T(0);
+          bi[q][u] = T(0);
+          if (q < nrhs && c < np) ld_xcd_c(vin + ((size_t)q * np + c) * 2, br[q][u], bi[q][u]);
         }
 #pragma unroll
       for (int q = 0; q < RP; ++q)
 #pragma unroll
         for (int u = 0; u < CU; ++u) {
           const int c = tid + u * NT;
-          if (q < nrhs && c < ny) {
-            const int e = q * nv + c;
-            const double ar = (double)yr[q][u], ai = (double)yi[q][u];
-            const double tr = cpr[c] * ar - cpi[c] * ai, ti = cpr[c] * ai + cpi[c] * ar;
-            if (dir == 0) { vr[e] = (T)((double)vr[e] - tr); vi[e] = (T)((double)vi[e] - ti); }
-            else { vr[e] = (T)tr; vi[e] = (T)ti; }
+          if (c < nv) {
+            vr[q * nv + c] = br[q][u];
+            vi[q * nv + c] = bi[q][u];
           }
         }
+      __syncthreads();
     }
-    __syncthreads();
-    // 4. partial row dot products for every (row, rhs) pair of this wave
+    // 3. partial row dot products for every (row, rhs) pair of this wave
     T acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = 0;
@@ -1301,17 +1314,23 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
       }
     }
     const T sum = wave_multi_sum(acc, lane);
+    // 4. this lane's entry of y_k (forward) or d_k (backward)
+    const T out = dir == 0 ? sum : yown - sum;
     if (my_store) {
-      if (dir == 0) {
-        st_xcd(my_y, sum);
-      } else {
-        const T d = yown - sum;
-        st_xcd(my_y, d);
-        if (my_a < ny) x[2 * ((size_t)my_q * M + (size_t)k * ny + my_a) + my_ri] += (double)d;
-      }
+      if (dir == 0) *my_y = out;   // read by the backward launch
+      else if (my_a < ny) x[2 * ((size_t)my_q * M + (size_t)k * ny + my_a) + my_ri] += (double)out;
     }
-    // 5. publish this block's rows: every wave's stores complete, then this
-    // workgroup's flag
+    // 5. publish the next block's input entry for this row:
+    //    forward v_{k+1} = r_{k+1} - XM_{k+1} y_k, backward w_{k-1} = XP_{k-1} d_k
+    const T other = __shfl_xor(out, 1 << SH);   // the other component of this (row, rhs)
+    if (my_store && has_next) {
+      const double ore = (double)(my_ri == 0 ? out : other), oim = (double)(my_ri == 0 ? other : out);
+      const cd t = cmul(cn, cd{ore, oim});
+      const cd v = dir == 0 ? csub(rn, t) : t;
+      T *vout = vb + (size_t)((s + 1) & 1) * RP * np * 2;
+      st_xcd(vout + ((size_t)my_q * np + my_a) * 2 + my_ri, (T)(my_ri == 0 ? v.r : v.i));
+    }
+    // 6. publish: every wave's stores complete, then this workgroup's flag
     stores_complete();
     __syncthreads();
     if (tid == 0)
@@ -1897,7 +1916,7 @@ void launch_psweep(int W, size_t lds, hipStream_t s, const T *G, const double *o
   static std::once_flag lds_attr;
   std::call_once(lds_attr, [] {
     (void)hipFuncSetAttribute((const void *)k_psweep<T, RP>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(2 * 768 * sizeof(double) + 2 * RP * 768 * sizeof(T)));
+                              (int)(2 * RP * 768 * sizeof(T)));
   });
   hipLaunchKernelGGL((k_psweep<T, RP>), dim3(W), dim3(NT), lds, s, G, op, r, y, x, nx, ny, np, nr, dir, sy,
                      g_spin_cap);
@@ -1916,7 +1935,7 @@ int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const d
     const int nr = std::min(sweep_rhs<T>(), nrhs - q0);
     const int rp = nr <= 1 ? 1 : nr <= 2 ? 2 : nr <= 4 ? 4 : 8;
     T *yq = (T *)sw.y + (size_t)2 * q0 * nx * np;
-    const size_t lds = (size_t)2 * np * sizeof(double) + (size_t)2 * rp * pad64(np) * sizeof(T);
+    const size_t lds = (size_t)2 * rp * pad64(np) * sizeof(T);
     // step flags start at zero; the abort flag is cleared by the caller, so
     // an abort stays visible across the sweeps of one solve attempt
     HIP_TRY(hipMemsetAsync(sw.sync->done, 0, sizeof sw.sync->done, s));
