@@ -1463,20 +1463,24 @@ __global__ void __launch_bounds__(NT) k_wrap_capacitance(fdfd_grid g, const doub
 // Gauss-Jordan inverse of K with partial pivoting, one launch per column j:
 // the launch reads the matrix before step j from one buffer and writes the
 // matrix after it to the other, so no element is read after it is written.
-// Workgroup (bx, by) owns rows [16 by, 16 by + 16) x columns [256 bx, +256)
-// and repeats the pivot search over column j (identical result everywhere).
-// Row interchanges are applied as the rows are written; the column swaps are
-// undone at the end (k_kgj_unpermute).
-constexpr int KGJ_ROWS = 16;
+// Workgroup (bx, by) owns rows [KGJ_ROWS by, +KGJ_ROWS) x columns
+// [256 bx, +256) and repeats the pivot search over column j (identical
+// result everywhere), reading it from a contiguous copy (colin) that the
+// previous launch wrote alongside column j of its output (colout gets
+// column j + 1 of this launch's output). Row interchanges are applied as the
+// rows are written; the column swaps are undone at the end
+// (k_kgj_unpermute).
+constexpr int KGJ_ROWS = 32;
 __global__ void __launch_bounds__(NT) k_kgj_col(const double *__restrict__ Ks, double *__restrict__ Kd, int N,
-                                                int j, int *piv, int *status) {
+                                                int j, const double *__restrict__ colin, double *__restrict__ colout,
+                                                int *piv, int *status) {
   __shared__ double bv[NT];
   __shared__ int bi[NT];
   const int tid = threadIdx.x;
   double best = -1;
   int arg = j;
   for (int i = j + tid; i < N; i += NT) {
-    const cd z = ld_c(Ks, (size_t)i * N + j);
+    const cd z = ld_c(colin, i);
     const double v = fabs(z.r) + fabs(z.i);
     if (v > best) { best = v; arg = i; }
   }
@@ -1492,7 +1496,7 @@ __global__ void __launch_bounds__(NT) k_kgj_col(const double *__restrict__ Ks, d
     __syncthreads();
   }
   const int p = bi[0];
-  const cd pv = ld_c(Ks, (size_t)p * N + j);
+  const cd pv = ld_c(colin, p);
   const cd d = (pv.r == 0 && pv.i == 0) ? cd{0, 0} : cinv(pv);
   if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
     piv[j] = p;
@@ -1509,12 +1513,19 @@ __global__ void __launch_bounds__(NT) k_kgj_col(const double *__restrict__ Ks, d
       v = c == j ? d : cmul(h, d);
     } else {
       const int sr = i == p ? j : i;   // row p now holds the old row j
-      const cd f = ld_c(Ks, (size_t)sr * N + j);
+      const cd f = ld_c(colin, sr);
       const cd g = cmul(f, d);
       v = c == j ? cd{-g.r, -g.i} : csub(ld_c(Ks, (size_t)sr * N + c), cmul(g, h));
     }
     st_c(Kd, (size_t)i * N + c, v);
+    if (c == j + 1) st_c(colout, i, v);
   }
+}
+
+// column 0 of K into the pivot-search copy
+__global__ void __launch_bounds__(NT) k_kgj_col0(const double *__restrict__ K, int N, double *__restrict__ col) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i < N) st_c(col, i, ld_c(K, (size_t)i * N));
 }
 
 __global__ void __launch_bounds__(NT) k_kgj_unpermute(double *K, int N, const int *__restrict__ piv) {
@@ -1896,14 +1907,17 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, h
   double *K = wrap_kinv(g, prec, factor);
   hipLaunchKernelGGL(k_wrap_capacitance<T>, dim3((N + NT - 1) / NT, N), dim3(NT), 0, s, *g, op, Glast, sig_prev,
                      z_prev, np, K);
-  // Gauss-Jordan: the pivot record in the (now free) pivot-inverse area,
-  // the ping-pong partner of K in the workspace's K scratch
-  int *piv = (int *)fw.pinv_all;
+  // Gauss-Jordan: two contiguous pivot-column copies and the pivot record in
+  // the (now free) pivot-inverse area, the ping-pong partner of K in the
+  // workspace's K scratch
+  double *colb[2] = {(double *)fw.pinv_all, (double *)fw.pinv_all + 2 * (size_t)N};
+  int *piv = (int *)((double *)fw.pinv_all + 4 * (size_t)N);
   double *Kb[2] = {K, fw.kscratch};
   const dim3 kgrid((N + NT - 1) / NT, (N + KGJ_ROWS - 1) / KGJ_ROWS);
+  hipLaunchKernelGGL(k_kgj_col0, dim3((N + NT - 1) / NT), dim3(NT), 0, s, (const double *)K, N, colb[0]);
   for (int j = 0; j < N; ++j)
-    hipLaunchKernelGGL(k_kgj_col, kgrid, dim3(NT), 0, s, (const double *)Kb[j & 1], Kb[(j + 1) & 1], N, j, piv,
-                       fw.status);
+    hipLaunchKernelGGL(k_kgj_col, kgrid, dim3(NT), 0, s, (const double *)Kb[j & 1], Kb[(j + 1) & 1], N, j,
+                       (const double *)colb[j & 1], colb[(j + 1) & 1], piv, fw.status);
   if (N & 1) HIP_TRY(hipMemcpyAsync(K, fw.kscratch, 16 * (size_t)N * N, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_kgj_unpermute, dim3((N + NT - 1) / NT), dim3(NT), 0, s, K, N, (const int *)piv);
   HIP_TRY(hipGetLastError());
