@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=64, help="requests per step")
     ap.add_argument("--rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--slots", type=int, default=16, help="requests in flight")
+    ap.add_argument("--engines", type=int, default=2, help="factorization engines (streams of step launches)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget of the timed CPU sample (sizes beyond it are extrapolated)")
@@ -238,10 +239,10 @@ def config_timings():
     return out
 
 
-def run_stream(reqs, precision, slots, tol):
+def run_stream(reqs, precision, slots, tol, engines=2):
     import torch
     from fdfdpy_amd.serving import ServingEngine, summarize
-    eng = ServingEngine("cuda:0", precision=precision, slots=slots, tol=tol)
+    eng = ServingEngine("cuda:0", precision=precision, slots=slots, tol=tol, engines=engines)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
 
@@ -251,7 +252,14 @@ def run_stream(reqs, precision, slots, tol):
     res = eng.run(reqs, progress=progress)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return summarize(res, wall), res
+    summ = summarize(res, wall)
+    if eng.factor_engines:
+        # batching of the factorizations: job-steps per shared step launch
+        steps = sum(r.problem.shape[0] * (-(-r.problem.shape[1] // 32)) + 1 for r in reqs)
+        launches = sum(e.launches for e in eng.factor_engines)
+        summ["engine_launches"] = launches
+        summ["jobs_per_launch"] = round(steps / launches, 3) if launches else None
+    return summ, res
 
 
 def main():
@@ -288,14 +296,14 @@ def main():
     for w in range(args.warmup):
         wreqs = [Request(i, C.c3_request(100000 + 8 * w + i), 0.0) for i in range(8)]
         for prec in (("c64",) if args.no_fp64 else ("c64", "fp64")):
-            s, _ = run_stream(wreqs, prec, args.slots, args.tol)
+            s, _ = run_stream(wreqs, prec, args.slots, args.tol, args.engines)
             log(f"warmup {w} {prec}: {s['solves']} solves in {s['wall_s']:.2f}s")
 
-    mixed, res_m = run_stream(reqs, "c64", args.slots, args.tol)
+    mixed, res_m = run_stream(reqs, "c64", args.slots, args.tol, args.engines)
     log(f"mixed: {json.dumps(mixed)}")
     fp64 = None
     if not args.no_fp64:
-        fp64, res_f = run_stream(reqs, "fp64", args.slots, args.tol)
+        fp64, res_f = run_stream(reqs, "fp64", args.slots, args.tol, args.engines)
         log(f"fp64: {json.dumps(fp64)}")
     roof = roofline_leg()
     log(f"roofline: {json.dumps(roof)}")
@@ -315,7 +323,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"serving stream: {n_req} Ez requests, grids 128-768, Poisson arrivals "
                                f"{args.rate}/s, 1-4 RHS each, residual {args.tol:g}",
-                   "requests": n_req, "batch": args.batch, "slots": args.slots, "parallelism": "single GPU"},
+                   "requests": n_req, "batch": args.batch, "slots": args.slots, "engines": args.engines,
+                   "parallelism": "single GPU"},
         "p50_latency_ms": round(mixed["p50_latency_s"] * 1e3, 3),
         "p99_latency_ms": round(mixed["p99_latency_s"] * 1e3, 3),
         "p50_service_ms": round(mixed["p50_service_s"] * 1e3, 3),

@@ -113,12 +113,14 @@ class ServingEngine:
     bytes for factors kept after their request completes. batch_factors:
     the factorizations of in-flight requests advance together in shared
     Gauss-Jordan step launches (the native engine) instead of one launch
-    sequence per request stream.
+    sequence per request stream. engines: number of native engines, each with
+    its own scheduler thread and stream; slot s uses engine s % engines, so
+    the step launches of different engines overlap on the device.
     """
 
     def __init__(self, device="cuda:0", precision="c64", slots=4, memory_budget=None,
                  tol=1e-10, max_iter=80, cache_bytes=0, keep_fields=False, serve_fn=None,
-                 batch_factors=True):
+                 batch_factors=True, engines=2):
         self.device = torch.device(device)
         self.precision = precision
         self.slots = slots
@@ -138,7 +140,15 @@ class ServingEngine:
         self.max_in_flight = 0
         self.max_bytes_in_flight = 0
         # factorizations of in-flight requests share step launches
-        self.factor_engine = FactorEngine(self.device) if (batch_factors and serve_fn is None) else None
+        self.factor_engines = ([FactorEngine(self.device) for _ in range(max(1, engines))]
+                               if (batch_factors and serve_fn is None) else [])
+        self.factor_engine = self.factor_engines[0] if self.factor_engines else None
+
+    def _engine_for(self, stream):
+        if not self.factor_engines:
+            return None
+        slot = next(i for i, st in enumerate(self.streams) if st is stream)
+        return self.factor_engines[slot % len(self.factor_engines)]
 
     # ------------------------------------------------------------ one request
     def _serve(self, req, stream):
@@ -151,7 +161,7 @@ class ServingEngine:
                 reused = True
             else:
                 op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, self.device)
-                op.engine = self.factor_engine
+                op.engine = self._engine_for(stream)
                 op.set_eps(p.eps)
             b = torch.as_tensor(p.rhs(), device=self.device)
             x, rep = op.solve(b, precision=self.precision, tol=self.tol, max_iter=self.max_iter)
