@@ -2329,6 +2329,7 @@ int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int pre
     if (!aborted) break;
     blockwise = true;  // persistent sweep could not run co-resident: redo with per-block launches
   }
+  st.restarts = blockwise ? 1 : 0;
   if (transpose) {
     if ((rc = fdfd_stretch_scale(g, x, nrhs, 0, stream))) return rc;
     HIP_TRY(hipStreamSynchronize(stream));

@@ -72,6 +72,7 @@ class SolveReport:
     max_rel_residual: float
     rel_residual: list
     fell_back: bool = False
+    restarts: int = 0           # persistent sweeps timed out; redone per block
     factor_seconds: float = 0.0
     solve_seconds: float = 0.0
     history: list = field(default_factory=list)
@@ -261,6 +262,7 @@ class DeviceOperator:
             rep2.fell_back = prec != precision or piv
             rep2.factor_seconds += rep.factor_seconds
             rep2.solve_seconds += rep.solve_seconds
+            rep2.restarts += rep.restarts
             rep = rep2
         if not rep.converged:
             raise L.NotConverged(L.FDFD_ERR_NOT_CONVERGED,
@@ -286,7 +288,7 @@ class DeviceOperator:
         if rc not in (L.FDFD_OK, L.FDFD_ERR_NOT_CONVERGED):
             L.check(rc)
         return SolveReport(precision=precision, iterations=st.iterations, converged=bool(st.converged),
-                           max_rel_residual=st.max_rel_residual, rel_residual=list(rel),
+                           max_rel_residual=st.max_rel_residual, rel_residual=list(rel), restarts=st.restarts,
                            factor_seconds=tf, solve_seconds=time.perf_counter() - t1)
 
     # -- stencil -------------------------------------------------------------

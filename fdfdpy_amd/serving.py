@@ -46,6 +46,7 @@ class Result:
     fell_back: bool
     factor_reused: bool
     error: str = ""
+    sweep_restarts: int = 0    # solves redone with per-block sweep launches
 
     @property
     def latency(self):
@@ -198,7 +199,7 @@ class ServingEngine:
                          x if self.keep_fields else None, req.arrival, admitted, done,
                          rep.precision if rep else self.precision, rep.iterations if rep else 0,
                          rep.max_rel_residual if rep else float("nan"), rep.fell_back if rep else False,
-                         reused, err)
+                         reused, err, rep.restarts if rep else 0)
             with cond:
                 results[req.rid] = res
                 state["used"] -= nbytes
@@ -252,4 +253,5 @@ def summarize(results, wall):
         "max_rel_residual": float(max((r.max_rel_residual for r in ok), default=float("nan"))),
         "fallbacks": sum(r.fell_back for r in ok),
         "mean_sweeps": float(np.mean([r.iterations for r in ok])) if ok else float("nan"),
+        "sweep_restarts": sum(r.sweep_restarts for r in ok),
     }

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define FDFD_ABI_VERSION 2
+#define FDFD_ABI_VERSION 3
 
 enum {
   FDFD_OK = 0,
@@ -73,11 +73,14 @@ typedef struct fdfd_grid {
 /* Refinement report of fdfd_solve (no reference counterpart: Pardiso's
  * solve returns no residual, linalg.py:137). */
 typedef struct fdfd_solve_stats {
-  int32_t iterations;     /* sweeps applied                                 */
+  int32_t iterations;     /* refinement steps applied                       */
   int32_t converged;      /* 1 when every RHS met the tolerance             */
   int32_t stalled;        /* 1 when refinement stopped contracting          */
   int32_t zero_rhs;       /* 1 when every RHS was zero (shortcut taken)     */
   double max_rel_residual;/* max over RHS of ||b - A x|| / ||b||            */
+  int32_t restarts;       /* 1 when the persistent sweeps timed out and the  */
+                          /* solve was redone with per-block launches       */
+  int32_t reserved;
 } fdfd_solve_stats;
 
 int fdfd_abi_version(void);
