@@ -1575,6 +1575,73 @@ __global__ void __launch_bounds__(NT) k_axpy(size_t len, const double *__restric
   for (size_t e = (size_t)blockIdx.x * NT + threadIdx.x; e < len; e += (size_t)gridDim.x * NT) y[e] += a[e];
 }
 
+// Truncated wrap (far corners dropped): K = [[I, B], [A, I]] with
+// B = G_{m-1} C2, A = Sigma_0 C1, so only S = I - A B (order n) is inverted,
+// and K^-1 [s1; s2] is w2 = S^-1 (s2 - A s1), w1 = s1 - B w2.
+// S from its negative product -A B (planar T, stride np) into a row-major
+// complex FP64 matrix of order n.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_wrap_schur(const T *__restrict__ negAB, int np, int n,
+                                                   double *__restrict__ S) {
+  const int j = blockIdx.x * NT + threadIdx.x, i = blockIdx.y;
+  if (j >= n) return;
+  const size_t e = (size_t)i * np + j;
+  cd v = {(double)negAB[e], (double)negAB[(size_t)np * np + e]};
+  if (i == j) v.r += 1.0;
+  st_c(S, (size_t)i * n + j, v);
+}
+
+// out[q][i] = base[q][i] + sgn sum_j M[i][j] c[j] v[q][j] (i < n), one wave per
+// (row, rhs). M is planar T with row stride ld (imaginary plane after ld^2
+// entries) or, for T = double with ld == 0, row-major complex FP64 of order n.
+// c (complex, optional) scales v; base (optional) is added.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_wrap_gemv(int n, const T *__restrict__ Mt, int ld,
+                                                  const double *__restrict__ c, const double *__restrict__ v,
+                                                  size_t vstride, const double *__restrict__ base, size_t bstride,
+                                                  double sgn, double *__restrict__ out, size_t ostride) {
+  const int lane = threadIdx.x & 63, i = blockIdx.x * (NT / 64) + (threadIdx.x >> 6), q = blockIdx.y;
+  if (i >= n) return;
+  const double *vq = v + vstride * q;
+  double sr = 0, si = 0;
+  for (int j = lane; j < n; j += 64) {
+    cd mv;
+    if (ld > 0) {
+      const size_t e = (size_t)i * ld + j;
+      mv = cd{(double)Mt[e], (double)Mt[(size_t)ld * ld + e]};
+    } else {
+      mv = ld_c(reinterpret_cast<const double *>(Mt), (size_t)i * n + j);
+    }
+    cd x = ld_c(vq, j);
+    if (c) x = cmul(ld_c(c, j), x);
+    sr += mv.r * x.r - mv.i * x.i;
+    si += mv.r * x.i + mv.i * x.r;
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    sr += __shfl_xor(sr, off);
+    si += __shfl_xor(si, off);
+  }
+  if (lane == 0) {
+    cd r = {sgn * sr, sgn * si};
+    if (base) r = cadd(r, ld_c(base + bstride * q, i));
+    st_c(out + ostride * q, i, r);
+  }
+}
+
+// f = -U C w for the truncated wrap: block 0 gets -XM_0 w1, block m-1 gets
+// -XP_{m-1} w2 (f is zero elsewhere)
+__global__ void __launch_bounds__(NT) k_wrap_f(fdfd_grid g, const double *__restrict__ op,
+                                               const double *__restrict__ w1, const double *__restrict__ w2,
+                                               double *__restrict__ f) {
+  const int n = g.ny, m = g.nx, i = blockIdx.x * NT + threadIdx.x, q = blockIdx.y;
+  if (i >= n) return;
+  const size_t M = (size_t)m * n;
+  const cd a = cmul(ld_c(op, FDFD_OP_XM * M + i), ld_c(w1 + 2 * (size_t)n * q, i));
+  const cd b = cmul(ld_c(op, FDFD_OP_XP * M + (size_t)(m - 1) * n + i), ld_c(w2 + 2 * (size_t)n * q, i));
+  st_c(f + 2 * M * q, i, cd{-a.r, -a.i});
+  st_c(f + 2 * M * q, (size_t)(m - 1) * n + i, cd{-b.r, -b.i});
+}
+
 // ------------------------------------------------------------------ norms
 // part[r][blk] = sum |w v|^2 over a slice; w = stretch weight when weighted.
 __device__ inline double stretch_abs2(const fdfd_grid &g, int i, int j) {
@@ -1744,7 +1811,8 @@ size_t factor_work_bytes(const fdfd_grid *g, int precision) {
 
 struct SolveWork {
   double *r; void *y; double *part; double *norms; SweepSync *sync;
-  double *t, *f;   // wrap correction: T^-1 r and -U C K^-1 V^T t
+  double *t, *f;         // wrap correction: T^-1 r and -U C K^-1 V^T t
+  double *u, *w1, *w2;   // truncated wrap: [nrhs][n] complex each
 };
 SolveWork solve_work_layout(const fdfd_grid *g, int precision, int nrhs, void *work) {
   const size_t M = (size_t)g->nx * g->ny, np = pad_n(g->ny), es = elem_size(precision);
@@ -1757,28 +1825,44 @@ SolveWork solve_work_layout(const fdfd_grid *g, int precision, int nrhs, void *w
   sw.norms = (double *)(base + off); off += align_up(8 * 4 * (size_t)nrhs, 256);
   sw.sync = (SweepSync *)(base + off); off += align_up(sizeof(SweepSync), 256);
   sw.t = (double *)(base + off);  off += align_up(16 * M * nrhs, 256);
-  sw.f = (double *)(base + off);
+  sw.f = (double *)(base + off);  off += align_up(16 * M * nrhs, 256);
+  const size_t wb = align_up(16 * (size_t)g->ny * nrhs, 256);
+  sw.u = (double *)(base + off);  off += wb;
+  sw.w1 = (double *)(base + off); off += wb;
+  sw.w2 = (double *)(base + off);
   return sw;
 }
 size_t solve_work_bytes(const fdfd_grid *g, int precision, int nrhs) {
   const size_t M = (size_t)g->nx * g->ny, np = pad_n(g->ny), es = elem_size(precision);
   return align_up(16 * M * nrhs, 256) + align_up(2 * es * (size_t)g->nx * np * nrhs, 256) +
          align_up(8 * (size_t)NORM_PARTS * nrhs * 2, 256) + align_up(8 * 4 * (size_t)nrhs, 256) +
-         align_up(sizeof(SweepSync), 256) + 2 * align_up(16 * M * nrhs, 256);
+         align_up(sizeof(SweepSync), 256) + 2 * align_up(16 * M * nrhs, 256) +
+         3 * align_up(16 * (size_t)g->ny * nrhs, 256);
 }
 
-// The factor buffer: the block inverses G_k ([m][2][np][np] of T), then
-// K^-1 of the wrap correction (row-major complex FP64, order 2n).
+// Exact wrap correction (full chains, K of order 2n) or the truncated one;
+// the factorization and the solve take the same decision.
+inline bool wrap_exact(const fdfd_grid *g) {
+  return g->npml_x < WRAP_MIN_PML || g->npml_y < WRAP_MIN_PML || 2 * g->npml_x + WRAP_EXTRA >= g->nx - 1;
+}
+// The factor buffer: the block inverses G_k ([m][2][np][np] of T), then the
+// wrap correction: K^-1 (row-major complex FP64, order 2n) when exact, else
+// S^-1 (order n) followed by a copy of Sigma_0 (planar T, np x np).
 size_t factor_blocks_bytes(const fdfd_grid *g, int precision) {
   const size_t np = pad_n(g->ny);
   return align_up((size_t)g->nx * 2 * np * np * elem_size(precision), 256);
 }
+size_t wrap_sinv_bytes(const fdfd_grid *g) { return align_up(16 * (size_t)g->ny * g->ny, 256); }
 size_t factor_total_bytes(const fdfd_grid *g, int precision) {
-  const size_t N = 2 * (size_t)g->ny;
-  return factor_blocks_bytes(g, precision) + 16 * N * N;
+  const size_t n = g->ny, np = pad_n(g->ny);
+  return factor_blocks_bytes(g, precision) +
+         std::max(64 * n * n, wrap_sinv_bytes(g) + 2 * np * np * elem_size(precision));
 }
 inline double *wrap_kinv(const fdfd_grid *g, int precision, const void *factor) {
   return (double *)((char *)factor + factor_blocks_bytes(g, precision));
+}
+inline void *wrap_sigma(const fdfd_grid *g, int precision, const void *factor) {
+  return (char *)factor + factor_blocks_bytes(g, precision) + wrap_sinv_bytes(g);
 }
 
 template <typename T, bool PIV>
@@ -1866,7 +1950,7 @@ int sweep_blockwise(const fdfd_grid *g, const double *op, const void *factor, co
 template <typename T>
 int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, hipStream_t s) {
   const int prec = sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64;
-  const int m = g->nx, n = g->ny, np = pad_n(n), nt = np / TB, N = 2 * n;
+  const int m = g->nx, n = g->ny, np = pad_n(n), nt = np / TB;
   const size_t np2 = (size_t)np * np, M = (size_t)m * n;
   FactorWork fw = factor_work_layout(g, prec, work);
   const T *G = (const T *)factor;
@@ -1874,7 +1958,7 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, h
   T *Z[2] = {(T *)fw.xcol, (T *)fw.xcol + 2 * np2};
   const T *Glast = G + (size_t)(m - 1) * 2 * np2;
   // exact chains over all blocks, or the truncated Sigma chain without Z
-  const bool exact = g->npml_x < WRAP_MIN_PML || g->npml_y < WRAP_MIN_PML || 2 * g->npml_x + WRAP_EXTRA >= m - 1;
+  const bool exact = wrap_exact(g);
   const int L = exact ? m - 1 : 2 * g->npml_x + WRAP_EXTRA;
   const T *sig_prev = G + (size_t)L * 2 * np2, *z_prev = exact ? Glast : nullptr;
   int zi = 0;
@@ -1905,8 +1989,26 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, h
   }
   HIP_TRY(hipGetLastError());
   double *K = wrap_kinv(g, prec, factor);
-  hipLaunchKernelGGL(k_wrap_capacitance<T>, dim3((N + NT - 1) / NT, N), dim3(NT), 0, s, *g, op, Glast, sig_prev,
-                     z_prev, np, K);
+  const int N = exact ? 2 * n : n;   // order of the matrix inverted below
+  if (exact) {
+    hipLaunchKernelGGL(k_wrap_capacitance<T>, dim3((N + NT - 1) / NT, N), dim3(NT), 0, s, *g, op, Glast, sig_prev,
+                       z_prev, np, K);
+  } else {
+    // S = I - Sigma_0 XM_0 G_{m-1} XP_{m-1} (order n), and a copy of Sigma_0
+    GemmBatch<T> b3;
+    memset(&b3, 0, sizeof b3);
+    b3.njobs = 1;
+    b3.job[0] = GemmJob<T>{sig_prev, Glast, nullptr, Q, op + 2 * (FDFD_OP_XM * M),
+                           op + 2 * (FDFD_OP_XP * M + (size_t)(m - 1) * n), T(-1)};
+    hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(NT), 0, s, b3, n, np);
+    hipLaunchKernelGGL(k_wrap_schur<T>, dim3((n + NT - 1) / NT, n), dim3(NT), 0, s, (const T *)Q, np, n, K);
+    HIP_TRY(hipMemcpyAsync(wrap_sigma(g, prec, factor), sig_prev, 2 * np2 * sizeof(T), hipMemcpyDeviceToDevice, s));
+    // the rest of the tail (sized for the exact form) is cleared, so a factor
+    // buffer is a deterministic function of the operator
+    const size_t used = factor_blocks_bytes(g, prec) + wrap_sinv_bytes(g) + 2 * np2 * sizeof(T);
+    const size_t total = factor_total_bytes(g, prec);
+    if (total > used) HIP_TRY(hipMemsetAsync((char *)factor + used, 0, total - used, s));
+  }
   // Gauss-Jordan: two contiguous pivot-column copies and the pivot record in
   // the (now free) pivot-inverse area, the ping-pong partner of K in the
   // workspace's K scratch
@@ -1920,6 +2022,34 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, h
                        (const double *)colb[j & 1], colb[(j + 1) & 1], piv, fw.status);
   if (N & 1) HIP_TRY(hipMemcpyAsync(K, fw.kscratch, 16 * (size_t)N * N, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_kgj_unpermute, dim3((N + NT - 1) / NT), dim3(NT), 0, s, K, N, (const int *)piv);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+// f = -U C K^-1 V^T t for the truncated wrap (K^-1 through S^-1, Sigma_0 and
+// G_{m-1}): u = t_0 - Sigma_0 (XM_0 t_{m-1}), w2 = S^-1 u,
+// w1 = t_{m-1} - G_{m-1} (XP_{m-1} w2), then f from w1, w2.
+template <typename T>
+int wrap_apply_truncated(const fdfd_grid *g, const double *op, const void *factor, const SolveWork &sw, int nrhs,
+                         hipStream_t s) {
+  const int prec = sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64;
+  const int m = g->nx, n = g->ny, np = pad_n(n);
+  const size_t M = (size_t)m * n, np2 = (size_t)np * np;
+  const T *sigma = (const T *)wrap_sigma(g, prec, factor);
+  const T *Glast = (const T *)factor + (size_t)(m - 1) * 2 * np2;
+  const double *sinv = wrap_kinv(g, prec, factor);
+  const double *XM0 = op + 2 * (FDFD_OP_XM * M), *XPl = op + 2 * (FDFD_OP_XP * M + (size_t)(m - 1) * n);
+  const double *t_last = sw.t + 2 * (size_t)(m - 1) * n;
+  const dim3 gg((n + NT / 64 - 1) / (NT / 64), nrhs);
+  hipLaunchKernelGGL(k_wrap_gemv<T>, gg, dim3(NT), 0, s, n, sigma, np, XM0, t_last, 2 * M,
+                     (const double *)sw.t, 2 * M, -1.0, sw.u, 2 * (size_t)n);
+  hipLaunchKernelGGL(k_wrap_gemv<double>, gg, dim3(NT), 0, s, n, sinv, 0, (const double *)nullptr,
+                     (const double *)sw.u, 2 * (size_t)n, (const double *)nullptr, (size_t)0, 1.0, sw.w2,
+                     2 * (size_t)n);
+  hipLaunchKernelGGL(k_wrap_gemv<T>, gg, dim3(NT), 0, s, n, Glast, np, XPl, (const double *)sw.w2,
+                     2 * (size_t)n, t_last, 2 * M, -1.0, sw.w1, 2 * (size_t)n);
+  hipLaunchKernelGGL(k_wrap_f, dim3((n + NT - 1) / NT, nrhs), dim3(NT), 0, s, *g, op, (const double *)sw.w1,
+                     (const double *)sw.w2, sw.f);
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
 }
@@ -2282,6 +2412,7 @@ int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int pre
     bool aborted = false;
     const bool fp64 = base_prec(precision) == FDFD_PREC_FP64;
     const double *kinv = wrap_kinv(g, precision, factor);
+    const bool exact = wrap_exact(g);
     for (int it = 1; it <= max_iter; ++it) {
       // t = T^-1 r; f = -U C K^-1 V^T t; t += T^-1 f; x += t  (x += A^-1 r)
       HIP_TRY(hipMemsetAsync(sw.t, 0, 16 * M * nrhs, stream));
@@ -2289,9 +2420,14 @@ int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int pre
                 : sweep_impl<float>(g, op, factor, rr, sw.t, nrhs, sw, stream, blockwise);
       if (rc) return rc;
       HIP_TRY(hipMemsetAsync(sw.f, 0, 16 * M * nrhs, stream));
-      hipLaunchKernelGGL(k_wrap_rhs, dim3((2 * g->ny + NT / 64 - 1) / (NT / 64), nrhs), dim3(NT), 0, stream, *g,
-                         op, kinv, (const double *)sw.t, sw.f);
-      HIP_TRY(hipGetLastError());
+      if (exact) {
+        hipLaunchKernelGGL(k_wrap_rhs, dim3((2 * g->ny + NT / 64 - 1) / (NT / 64), nrhs), dim3(NT), 0, stream, *g,
+                           op, kinv, (const double *)sw.t, sw.f);
+        HIP_TRY(hipGetLastError());
+      } else if ((rc = fp64 ? wrap_apply_truncated<double>(g, op, factor, sw, nrhs, stream)
+                            : wrap_apply_truncated<float>(g, op, factor, sw, nrhs, stream))) {
+        return rc;
+      }
       rc = fp64 ? sweep_impl<double>(g, op, factor, sw.f, sw.t, nrhs, sw, stream, blockwise)
                 : sweep_impl<float>(g, op, factor, sw.f, sw.t, nrhs, sw, stream, blockwise);
       if (rc) return rc;
