@@ -326,6 +326,20 @@ inline int batch_layout(FBatch<T> &b) {
   return wg;
 }
 
+// Four consecutive elements from LDS to global memory as 16-byte
+// non-temporal stores.
+__device__ inline void store4_nt(float *p, const float *v) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 x = {v[0], v[1], v[2], v[3]};
+  __builtin_nontemporal_store(x, reinterpret_cast<f4 *>(p));
+}
+__device__ inline void store4_nt(double *p, const double *v) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 x = {v[0], v[1]}, y = {v[2], v[3]};
+  __builtin_nontemporal_store(x, reinterpret_cast<d2 *>(p));
+  __builtin_nontemporal_store(y, reinterpret_cast<d2 *>(p + 2));
+}
+
 // Element (a, b) of the working matrix of block k before step p. For p == 0
 // this is S_k = T_k - l_k G_{k-1} u_{k-1} (identity on the padding).
 template <typename T>
@@ -757,25 +771,32 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     if (tj == p) cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Pr, L.Pi, qi, qj, lane, T(-1));
     else cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Rr, L.Ri, qi, qj, lane, T(-1));
   }
+  // the tile goes out through LDS as whole rows (128-byte lines, 16-byte
+  // stores) that bypass this XCD's L2: the next launch reads it from memory,
+  // possibly on another XCD
+  __syncthreads();  // all reads of Ar/Ai, Pr/Pi, Rr/Ri are done
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
-    const size_t idx = (size_t)(ti * TB + a) * J.np + tj * TB + b;
-    dst[idx] = cr[t];
-    dst[np2 + idx] = ci[t];
+    L.Ar[a][b] = cr[t];
+    L.Ai[a][b] = ci[t];
+  }
+  __syncthreads();
+  {
+    const int i = tid >> 3, c0 = (tid & 7) * 4;
+    const size_t idx = (size_t)(ti * TB + i) * J.np + tj * TB + c0;
+    store4_nt(dst + idx, &L.Ar[i][c0]);
+    store4_nt(dst + np2 + idx, &L.Ai[i][c0]);
   }
   const bool next_pivot = (p + 1 < nt) && ti == p + 1 && tj == p + 1;
   const bool next_block = (p == nt - 1) && ti == 0 && tj == 0 && k + 1 < J.m;
   if (next_pivot || next_block) {
-    __syncthreads();  // all reads of Ar/Ai, Pr/Pi, Rr/Ri are done
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
-      L.Ar[a][b] = cr[t];
-      L.Ai[a][b] = ci[t];
+    // the tile is in Ar/Ai; S_{k+1}[0,0] overwrites it only after every
+    // thread has read its row for the store above
+    if (next_block) {
+      __syncthreads();
+      schur_next_00(J, k, L);
     }
-    __syncthreads();
-    if (next_block) schur_next_00(J, k, L);
     tile_inverse<T, PIV>(L, J.status);
     store_pinv(L, Pnext);
   }
