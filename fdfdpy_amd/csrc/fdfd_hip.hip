@@ -376,11 +376,13 @@ __device__ inline void x_elem(const FJob<T> &J, int k, int p, const T *src, int 
   im = (T)vi;
 }
 
+// Six tiles of LDS per workgroup (34 KB -> 25 KB for complex64, 5 groups per
+// CU instead of 4): R = P X[p,J] replaces X[p,J] once every wave has read it,
+// and the tile inverse uses the P and X[I,p] buffers as its double buffer.
 template <typename T> struct Lds {
   T Pr[TB][TBP], Pi[TB][TBP];  // pivot inverse; inversion buffer 0
-  T Rr[TB][TBP], Ri[TB][TBP];  // P * X[p,J];   inversion buffer 1
-  T Ar[TB][TBP], Ai[TB][TBP];  // X[p,J]; tile to invert / inverse
-  T Br[TB][TBP], Bi[TB][TBP];  // X[I,p]
+  T Ar[TB][TBP], Ai[TB][TBP];  // X[p,J], then R = P X[p,J]; tile to invert / inverse
+  T Br[TB][TBP], Bi[TB][TBP];  // X[I,p]; inversion buffer 1
   T mag[2][TB];                // pivot candidates (double-buffered)
   int perm[TB];
   int colmap[TB];
@@ -453,8 +455,8 @@ __device__ void tile_inverse_diag(Lds<T> &L, int *status) {
   bool bad = false;
 #pragma unroll 2
   for (int j = 0; j < TB; j += 2) {
-    T (*Br)[TBP] = (j & 2) ? L.Rr : L.Pr;
-    T (*Bi)[TBP] = (j & 2) ? L.Ri : L.Pi;
+    T (*Br)[TBP] = (j & 2) ? L.Br : L.Pr;
+    T (*Bi)[TBP] = (j & 2) ? L.Bi : L.Pi;
 #pragma unroll
     for (int q = 0; q < 4; ++q) { Br[i][c0 + q] = ar[q]; Bi[i][c0 + q] = ai[q]; }
     __syncthreads();
@@ -520,7 +522,7 @@ __device__ void tile_inverse_diag(Lds<T> &L, int *status) {
 // Inverse of the TB x TB tile in (Ar, Ai), in place, by Gauss-Jordan with
 // partial pivoting inside the tile (the fallback factorization). Thread t keeps row t/8, columns
 // 4(t%8)..+3 in registers; each pivot step publishes the tile to one of two
-// LDS buffers (Pr/Pi, Rr/Ri, alternating), so one barrier per pivot orders
+// LDS buffers (Pr/Pi, Br/Bi, alternating), so one barrier per pivot orders
 // the exchange. The row interchanges become one permuted write-back.
 template <typename T>
 __device__ void tile_inverse_pivoted(Lds<T> &L, int *status) {
@@ -532,8 +534,8 @@ __device__ void tile_inverse_pivoted(Lds<T> &L, int *status) {
   for (int q = 0; q < 4; ++q) { ar[q] = L.Ar[i][c0 + q]; ai[q] = L.Ai[i][c0 + q]; }
   bool bad = false;
   for (int j = 0; j < TB; ++j) {
-    T (*Br)[TBP] = (j & 1) ? L.Rr : L.Pr;
-    T (*Bi)[TBP] = (j & 1) ? L.Ri : L.Pi;
+    T (*Br)[TBP] = (j & 1) ? L.Br : L.Pr;
+    T (*Bi)[TBP] = (j & 1) ? L.Bi : L.Pi;
 #pragma unroll
     for (int q = 0; q < 4; ++q) { Br[i][c0 + q] = ar[q]; Bi[i][c0 + q] = ai[q]; }
     if (j >= c0 && j < c0 + 4) {
@@ -616,11 +618,10 @@ __device__ inline void tile_inverse(Lds<T> &L, int *status) {
 
 template <typename T>
 __device__ inline void store_pinv(const Lds<T> &L, T *slot) {
-  for (int e = threadIdx.x; e < TB * TB; e += NT) {
-    const int a = e / TB, b = e % TB;
-    slot[e] = L.Ar[a][b];
-    slot[TB * TB + e] = L.Ai[a][b];
-  }
+  // whole rows, 16-byte non-temporal stores (read by the next launch)
+  const int i = threadIdx.x >> 3, c0 = (threadIdx.x & 7) * 4;
+  store4_nt(slot + i * TB + c0, &L.Ar[i][c0]);
+  store4_nt(slot + TB * TB + i * TB + c0, &L.Ai[i][c0]);
 }
 
 // S_{k+1}[0,0] tile from G_k[0,0] held in (Ar, Ai): T_{k+1} - l G u.
@@ -670,6 +671,9 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     local = r < c ? r : r + 1;
   }
   const FJob<T> &J = batch.job[jb];
+  // the critical workgroups bound the launch: their waves issue first on
+  // the CUs they share with the others
+  if ((int)blockIdx.x < batch.njobs) __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int qi = w >> 1, qj = w & 1;
   const size_t np2 = (size_t)J.np * J.np;
@@ -751,11 +755,12 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
   if (tj != p) {  // R = P X[p, J]
     typename Mfma<T>::v4 rr = {0, 0, 0, 0}, ri = {0, 0, 0, 0};
     cgemm_quad<T>(rr, ri, L.Pr, L.Pi, L.Ar, L.Ai, qi, qj, lane, T(1));
+    __syncthreads();  // every wave has read X[p, J]; R takes its place
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
-      L.Rr[a][b] = rr[t];
-      L.Ri[a][b] = ri[t];
+      L.Ar[a][b] = rr[t];
+      L.Ai[a][b] = ri[t];
     }
     if (ti == p) { cr = rr; ci = ri; }
     __syncthreads();
@@ -769,12 +774,12 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     }
   } else if (ti != p) {
     if (tj == p) cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Pr, L.Pi, qi, qj, lane, T(-1));
-    else cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Rr, L.Ri, qi, qj, lane, T(-1));
+    else cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Ar, L.Ai, qi, qj, lane, T(-1));
   }
   // the tile goes out through LDS as whole rows (128-byte lines, 16-byte
   // stores) that bypass this XCD's L2: the next launch reads it from memory,
   // possibly on another XCD
-  __syncthreads();  // all reads of Ar/Ai, Pr/Pi, Rr/Ri are done
+  __syncthreads();  // all reads of Ar/Ai, Pr/Pi, Br/Bi are done
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
@@ -984,9 +989,10 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
         __syncthreads();
         v4 rr = {0, 0, 0, 0}, ri = {0, 0, 0, 0};
         cgemm_quad<T>(rr, ri, L.Pr, L.Pi, L.Ar, L.Ai, qi, qj, lane, T(1));
-        acc_to_lds<T>(rr, ri, L.Rr, L.Ri, qi, qj, lane);
+        __syncthreads();  // every wave has read X[p, J]; R takes its place
+        acc_to_lds<T>(rr, ri, L.Ar, L.Ai, qi, qj, lane);
         __syncthreads();
-        cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Rr, L.Ri, qi, qj, lane, T(-1));
+        cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Ar, L.Ai, qi, qj, lane, T(-1));
       }
       __syncthreads();  // LDS is reused by the next step
     }
