@@ -55,7 +55,7 @@ def test_engine_factors_match_direct_factors():
     import threading
     from fdfdpy_amd.device import DeviceOperator, FactorEngine
     eng = FactorEngine("cuda:0")
-    probs = [C.c3_request(60 + i, n=n) for i, n in enumerate((128, 192, 256, 128))]
+    probs = [C.c3_request(60 + i, n=n) for i, n in enumerate((128, 192, 256, 160))]
     precs = ["c64", "fp64", "c64", "c64"]
     ops_a, ops_b = [], []
     for p in probs:
