@@ -79,6 +79,7 @@ SIGNATURES = {
     "fdfd_engine_wait": (c_int, [c_void_p, ctypes.c_uint64]),
     "fdfd_engine_launches": (ctypes.c_uint64, [c_void_p]),
     "fdfd_set_spin_cap": (None, [ctypes.c_uint]),
+    "fdfd_set_step_symmetry": (c_int, [c_int]),
 }
 
 _lib = None

@@ -235,6 +235,7 @@ __global__ void k_selftest_mfma_f32(const double *a, const double *b, double *d)
 // every later wait returns at once, so a grid always drains.
 constexpr unsigned SPIN_CAP = 1u << 21;
 unsigned g_spin_cap = SPIN_CAP;  // host setting passed to the persistent kernels
+int g_step_sym = 1;              // launch-per-step factorizations use symmetric steps
 
 template <typename T>
 __device__ inline T ld_xcd(const T *p) {
@@ -296,6 +297,8 @@ template <typename T> struct FJob {
   T *W;              // [2][np][np] ping-pong partner of G_k
   T *Pinv;           // [2 slots][2][TB][TB]
   int *status;       // set to 1 on a non-finite / zero pivot
+  T *dsym;           // [4][np] D, 1/D (re, im planes) of the block symmetrizer, written
+                     // by the initial-pivot launch; null: every tile of every step
   int m, n, np, nt;  // blocks, block size, padded block size, tiles per edge
   int k, p;          // step performed by this launch; k < 0: initial pivot
 };
@@ -309,11 +312,33 @@ template <typename T> struct FBatch {
 };
 // Tiles per launch of a job, and its critical tile: (p+1, p+1), or (0, 0)
 // at the last step of a block (and the only tile of the initial pivot).
+//
+// Symmetric steps. D A is complex symmetric for the diagonal stretch matrix D
+// (SC-PML), so D_k S_k is symmetric for every Schur complement S_k, and the
+// working matrix M of the Gauss-Jordan sweep keeps
+//   M[j,i] = s_i s_j (D_i / D_j) M[i,j],  s = -1 on pivots already taken, +1 else
+// (D restricted to one block: the ratio depends on y only). A job with dsym
+// set computes only the tiles I <= J of each step, reads X[p,J] (J < p) and
+// X[I,p] (I > p) through that relation, and at the last step of a block also
+// writes the tiles below the diagonal, so G_k is stored whole.
 template <typename T>
-__host__ __device__ inline int step_tiles(const FJob<T> &J) { return J.k < 0 ? 1 : J.nt * J.nt; }
+__host__ __device__ inline int step_tiles(const FJob<T> &J) {
+  return J.k < 0 ? 1 : (J.dsym ? J.nt * (J.nt + 1) / 2 : J.nt * J.nt);
+}
 template <typename T>
 __host__ __device__ inline int critical_tile(const FJob<T> &J) {
   return (J.k >= 0 && J.p + 1 < J.nt) ? (J.p + 1) * (J.nt + 1) : 0;
+}
+// Tile (I * nt + J) of the r-th non-critical workgroup of a job.
+template <typename T>
+__device__ inline int step_tile_of(const FJob<T> &J, int r) {
+  const int c = critical_tile(J), nt = J.nt;
+  if (!J.dsym) return r < c ? r : r + 1;
+  // tiles I <= J in row order; the critical tile (on the diagonal) skipped
+  const int ci = c / nt, cu = ci * nt - ci * (ci - 1) / 2;
+  int u = r < cu ? r : r + 1, I = 0;
+  while (u >= nt - I) { u -= nt - I; ++I; }
+  return I * nt + I + u;
 }
 template <typename T>
 inline int batch_layout(FBatch<T> &b) {
@@ -650,6 +675,34 @@ __device__ inline void schur_next_00(const FJob<T> &J, int k, Lds<T> &L) {
   __syncthreads();
 }
 
+// D over one block from the couplings of block 0: D_{y+1} / D_y =
+// A[y, y+1] / A[y+1, y] (the same ratio in every block); 1 on the padding.
+template <typename T>
+__device__ void block_symmetrizer(const FJob<T> &J) {
+  if (threadIdx.x != 0) return;
+  const int n = J.n, np = J.np;
+  const size_t M = (size_t)J.m * n;
+  cd d = {1.0, 0.0};
+  for (int y = 0; y < np; ++y) {
+    if (y >= n) d = {1.0, 0.0};
+    const cd di = cinv(d);
+    J.dsym[y] = (T)d.r;
+    J.dsym[np + y] = (T)d.i;
+    J.dsym[2 * np + y] = (T)di.r;
+    J.dsym[3 * np + y] = (T)di.i;
+    if (y + 1 < n)
+      d = cmul(d, cmul(ld_c(J.op, FDFD_OP_YP * M + y), cinv(ld_c(J.op, FDFD_OP_YM * M + y + 1))));
+  }
+}
+
+// sign * D[i] / D[j] of a symmetric job
+template <typename T>
+__device__ inline void sym_ratio(const T *dsym, int np, int i, int j, T sign, T &rr, T &ri) {
+  const T ar = dsym[i], ai = dsym[np + i], br = dsym[2 * np + j], bi = dsym[3 * np + j];
+  rr = sign * (ar * br - ai * bi);
+  ri = sign * (ar * bi + ai * br);
+}
+
 // One Gauss-Jordan step (k, p) for every job of the batch; one output tile
 // (I, J) per workgroup:
 //   I == p, J == p : P                     (P = inverse of X[p,p])
@@ -667,8 +720,7 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     local = critical_tile(batch.job[jb]);
   } else {
     while (jb + 1 < batch.njobs && (int)blockIdx.x >= batch.wg_start[jb + 1]) ++jb;
-    const int r = blockIdx.x - batch.wg_start[jb], c = critical_tile(batch.job[jb]);
-    local = r < c ? r : r + 1;
+    local = step_tile_of(batch.job[jb], blockIdx.x - batch.wg_start[jb]);
   }
   const FJob<T> &J = batch.job[jb];
   // the critical workgroups bound the launch: their waves issue first on
@@ -681,6 +733,7 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
 
   if (J.k < 0) {  // initial pivot: inverse of T_0[0,0]
     if (local == 0) {
+      if (J.dsym) block_symmetrizer(J);
       load_tile(J, 0, 0, (const T *)nullptr, 0, 0, L.Ar, L.Ai);
       __syncthreads();
       tile_inverse<T, PIV>(L, J.status);
@@ -703,13 +756,17 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     // plain loads of the previous launch's output: all of them before any use
     constexpr int PER = TB * TB / NT;
     T pr[PER], pim[PER], ar[PER], ai[PER], br[PER], bi[PER];
+    const bool mirA = J.dsym && tj < p, mirB = J.dsym && ti > p;
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int e = tid + u * NT, a = e / TB, b = e % TB;
       pr[u] = Pslot[e];
       pim[u] = Pslot[TB * TB + e];
-      const size_t ia = (size_t)(p * TB + a) * J.np + tj * TB + b;
-      const size_t ib = (size_t)(ti * TB + a) * J.np + p * TB + b;
+      // symmetric jobs read X[p,J] (J < p) as tile (J,p), X[I,p] (I > p) as (p,I)
+      const size_t ia = mirA ? (size_t)(tj * TB + a) * J.np + p * TB + b
+                             : (size_t)(p * TB + a) * J.np + tj * TB + b;
+      const size_t ib = mirB ? (size_t)(p * TB + a) * J.np + ti * TB + b
+                             : (size_t)(ti * TB + a) * J.np + p * TB + b;
       ar[u] = tj != p ? src[ia] : T(0);
       ai[u] = tj != p ? src[np2 + ia] : T(0);
       br[u] = ti != p ? src[ib] : T(0);
@@ -729,8 +786,24 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
       const int e = tid + u * NT, a = e / TB, b = e % TB;
       L.Pr[a][b] = pr[u];
       L.Pi[a][b] = pim[u];
-      if (tj != p) { L.Ar[a][b] = ar[u]; L.Ai[a][b] = ai[u]; }
-      if (ti != p) { L.Br[a][b] = br[u]; L.Bi[a][b] = bi[u]; }
+      if (mirA) {  // X[p,J][b][a] = -(D[J b'] / D[p a']) X[J,p][a][b]
+        T rr, ri;
+        sym_ratio(J.dsym, J.np, tj * TB + a, p * TB + b, T(-1), rr, ri);
+        L.Ar[b][a] = rr * ar[u] - ri * ai[u];
+        L.Ai[b][a] = rr * ai[u] + ri * ar[u];
+      } else if (tj != p) {
+        L.Ar[a][b] = ar[u];
+        L.Ai[a][b] = ai[u];
+      }
+      if (mirB) {  // X[I,p][b][a] = (D[p a'] / D[I b']) X[p,I][a][b]
+        T rr, ri;
+        sym_ratio(J.dsym, J.np, p * TB + a, ti * TB + b, T(1), rr, ri);
+        L.Br[b][a] = rr * br[u] - ri * bi[u];
+        L.Bi[b][a] = rr * bi[u] + ri * br[u];
+      } else if (ti != p) {
+        L.Br[a][b] = br[u];
+        L.Bi[a][b] = bi[u];
+      }
     }
   } else {
     // p == 0: the tiles of S_k = T_k - l_k G_{k-1} u_{k-1}, formed on load
@@ -792,6 +865,21 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     const size_t idx = (size_t)(ti * TB + i) * J.np + tj * TB + c0;
     store4_nt(dst + idx, &L.Ar[i][c0]);
     store4_nt(dst + np2 + idx, &L.Ai[i][c0]);
+    if (J.dsym && p == nt - 1 && ti != tj) {
+      // G_k is symmetric under D: G_k[J,I][i][c] = (D[I c] / D[J i]) G_k[I,J][c][i]
+      T mr[4], mi[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        T rr, ri;
+        sym_ratio(J.dsym, J.np, ti * TB + c0 + q, tj * TB + i, T(1), rr, ri);
+        const T vr = L.Ar[c0 + q][i], vi = L.Ai[c0 + q][i];
+        mr[q] = rr * vr - ri * vi;
+        mi[q] = rr * vi + ri * vr;
+      }
+      const size_t mdx = (size_t)(tj * TB + i) * J.np + ti * TB + c0;
+      store4_nt(dst + mdx, mr);
+      store4_nt(dst + np2 + mdx, mi);
+    }
   }
   const bool next_pivot = (p + 1 < nt) && ti == p + 1 && tj == p + 1;
   const bool next_block = (p == nt - 1) && ti == 0 && tj == 0 && k + 1 < J.m;
@@ -898,7 +986,7 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
   v4 cr, ci;
   // S_0 = T_0
   FJob<T> X;  // x_elem's view of the operator (p == 0 form)
-  X.op = J.op; X.G = J.G; X.m = J.m; X.n = J.n; X.np = J.np; X.nt = nt;
+  X.op = J.op; X.G = J.G; X.dsym = nullptr; X.m = J.m; X.n = J.n; X.np = J.np; X.nt = nt;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int a = ti * TB + qi * 16 + Mfma<T>::crow(lane, t), b = tj * TB + qj * 16 + (lane & 15);
@@ -1943,6 +2031,7 @@ int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, 
   b.njobs = 1;
   FJob<T> &J = b.job[0];
   J.op = op; J.G = (T *)factor; J.W = (T *)fw.W; J.Pinv = (T *)fw.Pinv; J.status = fw.status;
+  J.dsym = g_step_sym ? (T *)fw.xcol : nullptr;
   J.m = m; J.n = n; J.np = np; J.nt = nt;
   J.k = -1; J.p = 0;
   int rc = launch_step<T, PIV>(b, s);
@@ -2152,6 +2241,7 @@ struct EngineJob {
   void *work;
   int precision;               // FDFD_PREC_*
   bool pivot;                  // partial pivoting inside pivot tiles
+  bool sym = true;             // symmetric steps (upper tiles only)
   int m, n, np, nt;
   int k = -1, p = 0;          // next step; k = -1: initial pivot
   bool launched_all = false;  // every step is on the stream
@@ -2182,6 +2272,7 @@ template <typename T>
 void fill_job(FJob<T> &J, const EngineJob &e) {
   FactorWork fw = factor_work_layout(&e.g, e.precision, e.work);
   J.op = e.op; J.G = (T *)e.factor; J.W = (T *)fw.W; J.Pinv = (T *)fw.Pinv; J.status = fw.status;
+  J.dsym = e.sym ? (T *)fw.xcol : nullptr;
   J.m = e.m; J.n = e.n; J.np = e.np; J.nt = e.nt; J.k = e.k; J.p = e.p;
 }
 
@@ -2578,6 +2669,7 @@ int fdfd_engine_submit(fdfd_engine *E, const fdfd_grid *g, const double *op, voi
   e->g = *g; e->op = op; e->factor = factor; e->work = work;
   e->precision = base_prec(precision); e->pivot = precision & FDFD_FACTOR_PIVOT;
   e->m = g->nx; e->n = g->ny; e->np = pad_n(g->ny); e->nt = e->np / TB;
+  e->sym = g_step_sym != 0;
   HIP_TRY(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
   {
     std::lock_guard<std::mutex> lk(E->mu);
@@ -2614,6 +2706,12 @@ uint64_t fdfd_engine_launches(fdfd_engine *E) {
 }
 
 void fdfd_set_spin_cap(unsigned polls) { g_spin_cap = polls ? polls : SPIN_CAP; }
+
+int fdfd_set_step_symmetry(int on) {
+  const int prev = g_step_sym;
+  g_step_sym = on ? 1 : 0;
+  return prev;
+}
 
 int fdfd_selftest_mfma(int precision, const double *a, const double *b, double *d, hipStream_t stream) {
   if (precision < 0 || precision > 2) return fail(FDFD_ERR_ARG, "selftest mode must be 0, 1 or 2");

@@ -200,6 +200,17 @@ uint64_t fdfd_engine_launches(fdfd_engine *engine);
  * at once, which exercises the fallback paths (no reference counterpart). */
 void fdfd_set_spin_cap(unsigned polls);
 
+/* Launch-per-step factorizations (fdfd_factor with FDFD_FACTOR_STEPWISE or
+ * when the persistent launch does not fit, and the engine) use symmetric
+ * steps when on (the default): D A is complex symmetric for the SC-PML
+ * stretch matrix D, so each step computes only the tiles on and above
+ * the diagonal of the working matrix and reads the others through that
+ * symmetry; the result agrees with full steps to the factor's precision, not
+ * bit for bit. Off: every tile of every step (bit-identical to the persistent
+ * factorization). Applies to factorizations started afterwards; returns the
+ * previous setting. No reference counterpart. */
+int fdfd_set_step_symmetry(int on);
+
 /* Check of the MFMA operand layouts the factor kernels rely on (no reference
  * counterpart): D = A (16x4) * B (4x16) with the 16x16x4 f64 (precision 0)
  * or f32 (precision 1) instruction; a, b, d are row-major device arrays of

@@ -230,13 +230,52 @@ def test_persistent_and_stepwise_factors_identical(precision, n):
     """The single-launch factorization equals the launch-per-step one bit for bit."""
     p = C.c3_request(77, n=n)
     ops = []
-    for stepwise in (False, True):
-        op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, DEV)
-        op.set_eps(p.eps)
-        op.factor(precision, stepwise=stepwise)
-        ops.append(op)
-    torch.cuda.synchronize()
+    prev = L.lib().fdfd_set_step_symmetry(0)   # full steps: the persistent kernel's arithmetic
+    try:
+        for stepwise in (False, True):
+            op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, DEV)
+            op.set_eps(p.eps)
+            op.factor(precision, stepwise=stepwise)
+            ops.append(op)
+        torch.cuda.synchronize()
+    finally:
+        L.lib().fdfd_set_step_symmetry(prev)
     assert torch.equal(ops[0].factors[precision][0], ops[1].factors[precision][0])
+
+
+@pytest.mark.parametrize("precision,n,pol,npml", [("c64", 200, "Ez", None), ("fp64", 96, "Hz", None),
+                                                   ("c64", 128, "Hz", [12, 0]), ("fp64", 160, "Ez", [0, 14])])
+def test_symmetric_steps_match_full_steps(precision, n, pol, npml):
+    """Symmetric steps (upper tiles only, the rest through D A = (D A)^T)
+    give the full steps' block inverses to the factor's precision, and the
+    solve through them meets the oracle."""
+    lib = L.lib()
+    p = C.c3_request(78, n=n)
+    npml = p.npml if npml is None else npml
+    ops = []
+    for sym in (0, 1):
+        prev = lib.fdfd_set_step_symmetry(sym)
+        try:
+            op = DeviceOperator(p.shape, p.omega, p.dl, npml, pol, p.L0, DEV)
+            op.set_eps(p.eps)
+            op.factor(precision, stepwise=True)
+            ops.append(op)
+            torch.cuda.synchronize()
+        finally:
+            lib.fdfd_set_step_symmetry(prev)
+    # the block inverses G_k lead the factor buffer: [m][2][np][np] (square grid)
+    dt = torch.float32 if precision == "c64" else torch.float64
+    m, npad = p.shape[0], ((p.shape[1] + 31) // 32) * 32
+    full, sym = (o.factors[precision][0].view(dt)[: m * 2 * npad * npad].double() for o in ops)
+    d = ((sym - full).abs().max() / full.abs().max()).item()
+    assert d < (1e-3 if precision == "c64" else 1e-9), d
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(p.shape) + 1j * rng.standard_normal(p.shape)
+    x, rep = ops[1].solve(torch.tensor(b, device=DEV), precision=precision, tol=1e-12)
+    assert rep.converged and not rep.fell_back
+    sim = OracleSimulation(p.omega, p.eps, p.dl, npml, pol, p.L0)
+    xref = solver_direct(sim.A, b.reshape(-1)).reshape(p.shape)
+    assert relerr(x.cpu().numpy(), xref) < 1e-10
 
 
 def test_bounded_waits_fall_back_when_forced():

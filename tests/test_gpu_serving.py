@@ -51,7 +51,8 @@ def test_memory_budget_and_factor_cache():
 
 
 def test_engine_factors_match_direct_factors():
-    """Factorizations batched in shared step launches equal the one-by-one ones."""
+    """Factorizations batched in shared step launches equal the one-by-one
+    launch-per-step ones."""
     import threading
     from fdfdpy_amd.device import DeviceOperator, FactorEngine
     eng = FactorEngine("cuda:0")
@@ -65,7 +66,7 @@ def test_engine_factors_match_direct_factors():
             op.engine = e
             ops.append(op)
     for op, pr in zip(ops_a, precs):
-        op.factor(pr)
+        op.factor(pr, stepwise=True)
     threads = [threading.Thread(target=op.factor, args=(pr,)) for op, pr in zip(ops_b, precs)]
     for t in threads:
         t.start()
