@@ -297,6 +297,7 @@ template <typename T> struct FJob {
   T *W;              // [2][np][np] ping-pong partner of G_k
   T *Pinv;           // [2 slots][2][TB][TB]
   int *status;       // set to 1 on a non-finite / zero pivot
+  const T *dense;    // one dense matrix ([2][np][np]) to invert (m = 1); null: the operator's S_k
   T *dsym;           // [4][np] D, 1/D (re, im planes) of the block symmetrizer, written
                      // by the initial-pivot launch; null: every tile of every step
   int m, n, np, nt;  // blocks, block size, padded block size, tiles per edge
@@ -380,6 +381,11 @@ __device__ inline void x_elem(const FJob<T> &J, int k, int p, const T *src, int 
   if (a >= n || b >= n) {
     re = (a == b) ? T(1) : T(0);
     im = T(0);
+    return;
+  }
+  if (J.dense) {
+    re = J.dense[(size_t)a * J.np + b];
+    im = J.dense[np2 + (size_t)a * J.np + b];
     return;
   }
   const size_t M = (size_t)J.m * n;
@@ -986,7 +992,7 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
   v4 cr, ci;
   // S_0 = T_0
   FJob<T> X;  // x_elem's view of the operator (p == 0 form)
-  X.op = J.op; X.G = J.G; X.dsym = nullptr; X.m = J.m; X.n = J.n; X.np = J.np; X.nt = nt;
+  X.op = J.op; X.G = J.G; X.dense = nullptr; X.dsym = nullptr; X.m = J.m; X.n = J.n; X.np = J.np; X.nt = nt;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int a = ti * TB + qi * 16 + Mfma<T>::crow(lane, t), b = tj * TB + qj * 16 + (lane & 15);
@@ -1693,14 +1699,24 @@ __global__ void __launch_bounds__(NT) k_axpy(size_t len, const double *__restric
 // Truncated wrap (far corners dropped): K = [[I, B], [A, I]] with
 // B = G_{m-1} C2, A = Sigma_0 C1, so only S = I - A B (order n) is inverted,
 // and K^-1 [s1; s2] is w2 = S^-1 (s2 - A s1), w1 = s1 - B w2.
-// S from its negative product -A B (planar T, stride np) into a row-major
-// complex FP64 matrix of order n.
+// S from its negative product -A B (planar T, stride np): planar complex
+// FP64 ([2][np][np], identity on the padding) when tiled, else row-major
+// complex FP64 of order n.
 template <typename T>
 __global__ void __launch_bounds__(NT) k_wrap_schur(const T *__restrict__ negAB, int np, int n,
-                                                   double *__restrict__ S) {
+                                                   double *__restrict__ S, int tiled) {
   const int j = blockIdx.x * NT + threadIdx.x, i = blockIdx.y;
-  if (j >= n) return;
   const size_t e = (size_t)i * np + j;
+  if (tiled) {
+    if (j >= np) return;
+    cd v = {0, 0};
+    if (i < n && j < n) v = cd{(double)negAB[e], (double)negAB[(size_t)np * np + e]};
+    if (i == j) v.r += 1.0;
+    S[e] = v.r;
+    S[(size_t)np * np + e] = v.i;
+    return;
+  }
+  if (j >= n) return;
   cd v = {(double)negAB[e], (double)negAB[(size_t)np * np + e]};
   if (i == j) v.r += 1.0;
   st_c(S, (size_t)i * n + j, v);
@@ -1962,12 +1978,24 @@ inline bool wrap_exact(const fdfd_grid *g) {
 }
 // The factor buffer: the block inverses G_k ([m][2][np][np] of T), then the
 // wrap correction: K^-1 (row-major complex FP64, order 2n) when exact, else
-// S^-1 (order n) followed by a copy of Sigma_0 (planar T, np x np).
+// S^-1 (order n, planar and padded to np when tiled) followed by a copy of
+// Sigma_0 (planar T, np x np).
 size_t factor_blocks_bytes(const fdfd_grid *g, int precision) {
   const size_t np = pad_n(g->ny);
   return align_up((size_t)g->nx * 2 * np * np * elem_size(precision), 256);
 }
-size_t wrap_sinv_bytes(const fdfd_grid *g) { return align_up(16 * (size_t)g->ny * g->ny, 256); }
+size_t wrap_sinv_bytes(const fdfd_grid *g) {
+  const size_t np = pad_n(g->ny);
+  return align_up(16 * np * np, 256);
+}
+// S^-1 of the truncated wrap is formed by the tiled Gauss-Jordan steps (planar
+// [2][np][np] complex FP64) when their two pivot-inverse slots fit the
+// workspace's exchange area and S fits the K scratch, else by the
+// column-by-column kernels (row-major).
+inline bool wrap_sinv_tiled(const fdfd_grid *g, int precision) {
+  const int np = pad_n(g->ny);
+  return np <= 2 * g->ny && (precision == FDFD_PREC_FP64 || np / TB >= 2);
+}
 size_t factor_total_bytes(const fdfd_grid *g, int precision) {
   const size_t n = g->ny, np = pad_n(g->ny);
   return factor_blocks_bytes(g, precision) +
@@ -2064,7 +2092,7 @@ int sweep_blockwise(const fdfd_grid *g, const double *op, const void *factor, co
 // of K). Temporaries live in the factorization workspace, whose exchange
 // buffers are free once the block inverses are complete.
 template <typename T>
-int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, hipStream_t s) {
+int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, bool pivoted, hipStream_t s) {
   const int prec = sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64;
   const int m = g->nx, n = g->ny, np = pad_n(n), nt = np / TB;
   const size_t np2 = (size_t)np * np, M = (size_t)m * n;
@@ -2117,13 +2145,40 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, h
     b3.job[0] = GemmJob<T>{sig_prev, Glast, nullptr, Q, op + 2 * (FDFD_OP_XM * M),
                            op + 2 * (FDFD_OP_XP * M + (size_t)(m - 1) * n), T(-1)};
     hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(NT), 0, s, b3, n, np);
-    hipLaunchKernelGGL(k_wrap_schur<T>, dim3((n + NT - 1) / NT, n), dim3(NT), 0, s, (const T *)Q, np, n, K);
+    const bool tiled = wrap_sinv_tiled(g, prec);
+    if (tiled) {
+      // S into the K scratch; its inverse into the S^-1 slot of the factor
+      hipLaunchKernelGGL(k_wrap_schur<T>, dim3((np + NT - 1) / NT, np), dim3(NT), 0, s, (const T *)Q, np, n,
+                         fw.kscratch, 1);
+    } else {
+      HIP_TRY(hipMemsetAsync(K, 0, wrap_sinv_bytes(g), s));
+      hipLaunchKernelGGL(k_wrap_schur<T>, dim3((n + NT - 1) / NT, n), dim3(NT), 0, s, (const T *)Q, np, n, K, 0);
+    }
     HIP_TRY(hipMemcpyAsync(wrap_sigma(g, prec, factor), sig_prev, 2 * np2 * sizeof(T), hipMemcpyDeviceToDevice, s));
     // the rest of the tail (sized for the exact form) is cleared, so a factor
     // buffer is a deterministic function of the operator
     const size_t used = factor_blocks_bytes(g, prec) + wrap_sinv_bytes(g) + 2 * np2 * sizeof(T);
     const size_t total = factor_total_bytes(g, prec);
     if (total > used) HIP_TRY(hipMemsetAsync((char *)factor + used, 0, total - used, s));
+    if (tiled) {
+      // np/32 + 1 step launches of the block factorization's kernel on one
+      // dense FP64 job: diagonal 32 x 32 pivot tiles (S is a small
+      // perturbation of I), partial pivoting inside them for a pivoted factor
+      FBatch<double> b;
+      memset(&b, 0, sizeof b);
+      b.njobs = 1;
+      FJob<double> &J = b.job[0];
+      J.op = op; J.G = K; J.W = (double *)fw.W; J.Pinv = (double *)fw.pinv_all; J.status = fw.status;
+      J.dense = fw.kscratch; J.dsym = nullptr;
+      J.m = 1; J.n = n; J.np = np; J.nt = nt;
+      J.k = -1; J.p = 0;
+      int rc = pivoted ? launch_step<double, true>(b, s) : launch_step<double, false>(b, s);
+      for (int p = 0; p < nt && !rc; ++p) {
+        J.k = 0; J.p = p;
+        rc = pivoted ? launch_step<double, true>(b, s) : launch_step<double, false>(b, s);
+      }
+      return rc;
+    }
   }
   // Gauss-Jordan: two contiguous pivot-column copies and the pivot record in
   // the (now free) pivot-inverse area, the ping-pong partner of K in the
@@ -2159,7 +2214,8 @@ int wrap_apply_truncated(const fdfd_grid *g, const double *op, const void *facto
   const dim3 gg((n + NT / 64 - 1) / (NT / 64), nrhs);
   hipLaunchKernelGGL(k_wrap_gemv<T>, gg, dim3(NT), 0, s, n, sigma, np, XM0, t_last, 2 * M,
                      (const double *)sw.t, 2 * M, -1.0, sw.u, 2 * (size_t)n);
-  hipLaunchKernelGGL(k_wrap_gemv<double>, gg, dim3(NT), 0, s, n, sinv, 0, (const double *)nullptr,
+  hipLaunchKernelGGL(k_wrap_gemv<double>, gg, dim3(NT), 0, s, n, sinv, wrap_sinv_tiled(g, prec) ? np : 0,
+                     (const double *)nullptr,
                      (const double *)sw.u, 2 * (size_t)n, (const double *)nullptr, (size_t)0, 1.0, sw.w2,
                      2 * (size_t)n);
   hipLaunchKernelGGL(k_wrap_gemv<T>, gg, dim3(NT), 0, s, n, Glast, np, XPl, (const double *)sw.w2,
@@ -2426,7 +2482,8 @@ int fdfd_factor(const fdfd_grid *g, const double *op, void *factor, int precisio
                          : factor_impl<float, false>(g, op, factor, work, stream));
     if (rc) return rc;
   }
-  return fp64 ? wrap_setup<double>(g, op, factor, work, stream) : wrap_setup<float>(g, op, factor, work, stream);
+  return fp64 ? wrap_setup<double>(g, op, factor, work, piv, stream)
+              : wrap_setup<float>(g, op, factor, work, piv, stream);
 }
 
 int fdfd_factor_wrap(const fdfd_grid *g, const double *op, void *factor, int precision, void *work,
@@ -2434,8 +2491,9 @@ int fdfd_factor_wrap(const fdfd_grid *g, const double *op, void *factor, int pre
   if (int rc = check_grid(g)) return rc;
   if (int rc = check_prec(precision)) return rc;
   if (!op || !factor || !work) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_factor_wrap");
-  return base_prec(precision) == FDFD_PREC_FP64 ? wrap_setup<double>(g, op, factor, work, stream)
-                                                : wrap_setup<float>(g, op, factor, work, stream);
+  const bool piv = precision & FDFD_FACTOR_PIVOT;
+  return base_prec(precision) == FDFD_PREC_FP64 ? wrap_setup<double>(g, op, factor, work, piv, stream)
+                                                : wrap_setup<float>(g, op, factor, work, piv, stream);
 }
 
 int fdfd_factor_status(const fdfd_grid *g, int precision, void *work, hipStream_t stream) {
