@@ -1499,47 +1499,81 @@ template <typename T> struct GemmBatch {
   GemmJob<T> job[2];
 };
 
+// k tile kt of the operands of output tile (ti, tj): A[ti, kt] and the
+// scaled B[kt, tj], four elements of each plane per thread (zeros past nt)
 template <typename T>
-__global__ void __launch_bounds__(NT) k_cgemm(GemmBatch<T> batch, int n, int np) {
-  __shared__ T Ar[TB][TBP], Ai[TB][TBP], Br[TB][TBP], Bi[TB][TBP];
+__device__ __forceinline__ void gemm_fetch(const GemmJob<T> &J, int n, int np, int nt, int ti, int tj, int kt,
+                                           int tid, T (&ar)[TB * TB / NT], T (&ai)[TB * TB / NT],
+                                           T (&br)[TB * TB / NT], T (&bi)[TB * TB / NT]) {
+  const size_t np2 = (size_t)np * np;
+  const bool live = kt < nt;
+#pragma unroll
+  for (int u = 0; u < TB * TB / NT; ++u) {
+    const int e = tid + u * NT, a = e / TB, b = e % TB;
+    const int brow = kt * TB + a, bcol = tj * TB + b;
+    const size_t ia = (size_t)(ti * TB + a) * np + kt * TB + b, ib = (size_t)brow * np + bcol;
+    ar[u] = live ? J.A[ia] : T(0);
+    ai[u] = live ? J.A[np2 + ia] : T(0);
+    double xr = live ? (double)J.B[ib] : 0.0, xi = live ? (double)J.B[np2 + ib] : 0.0;
+    if (J.rs && live) {
+      const cd sc = brow < n ? ld_c(J.rs, brow) : cd{0, 0};
+      const double yr = sc.r * xr - sc.i * xi, yi = sc.r * xi + sc.i * xr;
+      xr = yr; xi = yi;
+    }
+    if (J.cs && live) {
+      const cd sc = bcol < n ? ld_c(J.cs, bcol) : cd{0, 0};
+      const double yr = sc.r * xr - sc.i * xi, yi = sc.r * xi + sc.i * xr;
+      xr = yr; xi = yi;
+    }
+    br[u] = (T)xr;
+    bi[u] = (T)xi;
+  }
+}
+
+// Output tile (ti, tj) of job jb per workgroup of GT threads: two groups of
+// four waves take alternate k tiles; each group keeps the loads of its next
+// tile in flight while it multiplies the current one, and group 1's partial
+// sums are added to group 0's in a fixed order at the end.
+constexpr int GT = 2 * NT;
+template <typename T>
+__global__ void __launch_bounds__(GT) k_cgemm(GemmBatch<T> batch, int n, int np) {
+  __shared__ T Ar[2][TB][TBP], Ai[2][TB][TBP], Br[2][TB][TBP], Bi[2][TB][TBP];
   const int nt = np / TB;
   const int jb = blockIdx.x / (nt * nt), t = blockIdx.x % (nt * nt);
   const int ti = t / nt, tj = t % nt;
   const GemmJob<T> J = jb == 0 ? batch.job[0] : batch.job[1];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, qi = wv >> 1, qj = wv & 1;
+  const int grp = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const int lane = tid & 63, wv = tid >> 6, qi = wv >> 1, qj = wv & 1;
   const size_t np2 = (size_t)np * np;
+  constexpr int PER = TB * TB / NT;
+  T ar[PER], ai[PER], br[PER], bi[PER];
   typename Mfma<T>::v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-  for (int kt = 0; kt < nt; ++kt) {
-    for (int e = tid; e < TB * TB; e += NT) {
-      const int a = e / TB, b = e % TB;
-      const size_t ia = (size_t)(ti * TB + a) * np + kt * TB + b;
-      Ar[a][b] = J.A[ia];
-      Ai[a][b] = J.A[np2 + ia];
-      const int br = kt * TB + a, bc = tj * TB + b;
-      const size_t ib = (size_t)br * np + bc;
-      double xr = (double)J.B[ib], xi = (double)J.B[np2 + ib];
-      if (J.rs) {
-        const cd s = br < n ? ld_c(J.rs, br) : cd{0, 0};
-        const double yr = s.r * xr - s.i * xi, yi = s.r * xi + s.i * xr;
-        xr = yr; xi = yi;
-      }
-      if (J.cs) {
-        const cd s = bc < n ? ld_c(J.cs, bc) : cd{0, 0};
-        const double yr = s.r * xr - s.i * xi, yi = s.r * xi + s.i * xr;
-        xr = yr; xi = yi;
-      }
-      Br[a][b] = (T)xr;
-      Bi[a][b] = (T)xi;
+  const int iters = (nt + 1) / 2;
+  gemm_fetch<T>(J, n, np, nt, ti, tj, grp, tid, ar, ai, br, bi);
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + u * NT, a = e / TB, b = e % TB;
+      Ar[grp][a][b] = ar[u];
+      Ai[grp][a][b] = ai[u];
+      Br[grp][a][b] = br[u];
+      Bi[grp][a][b] = bi[u];
     }
     __syncthreads();
-    cgemm_quad<T>(cr, ci, Ar, Ai, Br, Bi, qi, qj, lane, J.sign);
+    const int kt = 2 * it + grp;
+    if (it + 1 < iters) gemm_fetch<T>(J, n, np, nt, ti, tj, kt + 2, tid, ar, ai, br, bi);
+    if (kt < nt) cgemm_quad<T>(cr, ci, Ar[grp], Ai[grp], Br[grp], Bi[grp], qi, qj, lane, J.sign);
     __syncthreads();
   }
+  if (grp == 1) acc_to_lds<T>(cr, ci, Ar[0], Ai[0], qi, qj, lane);
+  __syncthreads();
+  if (grp == 1) return;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int row = ti * TB + qi * 16 + Mfma<T>::crow(lane, q), col = tj * TB + qj * 16 + (lane & 15);
+    const int a = qi * 16 + Mfma<T>::crow(lane, q), b = qj * 16 + (lane & 15);
+    const int row = ti * TB + a, col = tj * TB + b;
     const size_t idx = (size_t)row * np + col;
-    T vr = cr[q], vi = ci[q];
+    T vr = cr[q] + Ar[0][a][b], vi = ci[q] + Ai[0][a][b];
     if (J.D) { vr += J.D[idx]; vi += J.D[np2 + idx]; }
     J.C[idx] = vr;
     J.C[np2 + idx] = vi;
@@ -2118,13 +2152,13 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
     if (exact)
       b1.job[1] = GemmJob<T>{Gk, z_prev, nullptr, Z[zi], op + 2 * (FDFD_OP_XP * M + (size_t)k * n), nullptr,
                              T(-1)};
-    hipLaunchKernelGGL(k_cgemm<T>, dim3(b1.njobs * nt * nt), dim3(NT), 0, s, b1, n, np);
+    hipLaunchKernelGGL(k_cgemm<T>, dim3(b1.njobs * nt * nt), dim3(GT), 0, s, b1, n, np);
     // Sigma_k = G_k + Q G_k
     GemmBatch<T> b2;
     memset(&b2, 0, sizeof b2);
     b2.njobs = 1;
     b2.job[0] = GemmJob<T>{Q, Gk, Gk, Sig, nullptr, nullptr, T(1)};
-    hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(NT), 0, s, b2, n, np);
+    hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(GT), 0, s, b2, n, np);
     sig_prev = Sig;
     if (exact) {
       z_prev = Z[zi];
@@ -2144,7 +2178,7 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
     b3.njobs = 1;
     b3.job[0] = GemmJob<T>{sig_prev, Glast, nullptr, Q, op + 2 * (FDFD_OP_XM * M),
                            op + 2 * (FDFD_OP_XP * M + (size_t)(m - 1) * n), T(-1)};
-    hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(NT), 0, s, b3, n, np);
+    hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(GT), 0, s, b3, n, np);
     const bool tiled = wrap_sinv_tiled(g, prec);
     if (tiled) {
       // S into the K scratch; its inverse into the S^-1 slot of the factor
