@@ -30,11 +30,11 @@ sys.path.insert(0, ROOT)
 
 FP32_MFMA_PEAK_TFLOPS = 157.3
 FP64_MFMA_PEAK_TFLOPS = 78.6
-# HBM bytes per k_gj_step<float, false> launch at n = 512 (FETCH_SIZE x 2.000 +
-# WRITE_SIZE x 1.000, calibrated on a 256 MiB copy), from the rocprofv3 --pmc
-# runs of `bench.py --roofline-only` summarized in
+# HBM bytes per k_gj_step<float, false> launch at n = 512 with symmetric steps
+# (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, calibrated on a 256 MiB copy),
+# from the rocprofv3 --pmc runs of `bench.py --roofline-only` summarized in
 # profiles/k_gj_step_c64_512_roofline.md
-GJ_STEP_C64_512_TRAFFIC_BYTES = 5.44e6
+GJ_STEP_C64_512_TRAFFIC_BYTES = 3.461e6
 
 
 def log(*a):
@@ -77,20 +77,45 @@ def _timed_factor(op, precision, stepwise):
     return e0.elapsed_time(e1)
 
 
+def _timed_wrap(op, precision):
+    """The x-wrap correction step alone (fdfd_factor_wrap) on op's resident
+    factor, timed with HIP events on its stream."""
+    import ctypes
+    import torch
+    from fdfdpy_amd import _lib as L
+    from fdfdpy_amd.device import _stream, _vp
+    lib = L.lib()
+    g = ctypes.byref(op.grid)
+    fac = op.factors[precision][0]
+    work = torch.empty(lib.fdfd_factor_workspace_bytes(g, L.PREC[precision]), dtype=torch.uint8, device="cuda:0")
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(s)
+    L.check(lib.fdfd_factor_wrap(g, _vp(op.op), _vp(fac), L.PREC[precision], _vp(work), _stream("cuda:0")))
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1)
+
+
 def roofline_leg(n=512, precision="c64"):
     """Factor one n x n operator with the stepwise kernel the request loop uses.
 
     The factorization is n*(n/32) back-to-back k_gj_step launches (+1 for
-    the first pivot), timed with HIP events on their stream. The single
-    persistent launch (k_gj_persist, the single-request path) is timed too.
+    the first pivot), timed with HIP events on their stream; the x-wrap
+    correction that follows them in fdfd_factor is timed alone and taken out.
+    The single persistent launch (k_gj_persist, the single-request path) is
+    timed too.
     """
     from fdfdpy_amd import configs as C
     from fdfdpy_amd.device import DeviceOperator
     p = C.c3_request(7, n=n)
     op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, "cuda:0")
     op.set_eps(p.eps)
-    ms = _timed_factor(op, precision, stepwise=True)
-    ms_persist = _timed_factor(op, precision, stepwise=False)
+    ms_all = _timed_factor(op, precision, stepwise=True)
+    ms_wrap = _timed_wrap(op, precision)
+    ms = ms_all - ms_wrap
+    ms_persist = _timed_factor(op, precision, stepwise=False) - ms_wrap
     nt = (n + 31) // 32
     launches = n * nt + 1
     flops = 8.0 * n ** 4                       # 8 m n^3 real flops, m = n blocks
@@ -105,6 +130,7 @@ def roofline_leg(n=512, precision="c64"):
         "workload": f"complex factor of a {n}x{n} Ez operator: {launches} launches, "
                     f"8*n^4 = {flops:.3e} flops per factor, 8*n^2*32 = {8.0 * n * n * 32:.3e} per launch",
         "avg_launch_us": round(ms * 1e3 / launches, 3), "factor_ms": round(ms, 3),
+        "wrap_ms": round(ms_wrap, 3),
         "persistent_factor_ms": round(ms_persist, 3),
         "persistent_achieved_tflops": round(flops / (ms_persist * 1e-3) / 1e12, 3),
     }
