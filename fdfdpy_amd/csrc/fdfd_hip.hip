@@ -304,11 +304,17 @@ template <typename T> struct FJob {
   int k, p;          // step performed by this launch; k < 0: initial pivot
 };
 // Workgroup b < njobs takes the critical tile of job b (the tile inverted for
-// the next step), so those start first; the other tiles of job j are
-// workgroups [wg_start[j], wg_start[j + 1]), with wg_start[0] = njobs.
+// the next step), so those start first; the rest of the first crow * XCDS
+// workgroups idle. The other tiles of all jobs, concatenated (job j's are
+// [first[j], first[j + 1]), in row order), are cut into XCDS equal ranges and
+// range x goes to the workgroups b = XCDS * (crow + i) + x, which the
+// dispatcher places on XCD x: each XCD reads a job's pivot-row and pivot-
+// column tiles into its own L2 only for the jobs (or rows of a job) it holds.
+constexpr int XCDS = 8;
 template <typename T> struct FBatch {
   int njobs;
-  int wg_start[MAXJ + 1];
+  int crow;                // ceil(njobs / XCDS)
+  int first[MAXJ + 1];
   FJob<T> job[MAXJ];
 };
 // Tiles per launch of a job, and its critical tile: (p+1, p+1), or (0, 0)
@@ -343,13 +349,14 @@ __device__ inline int step_tile_of(const FJob<T> &J, int r) {
 }
 template <typename T>
 inline int batch_layout(FBatch<T> &b) {
-  int wg = b.njobs;
+  int w = 0;
   for (int j = 0; j < b.njobs; ++j) {
-    b.wg_start[j] = wg;
-    wg += step_tiles(b.job[j]) - 1;
+    b.first[j] = w;
+    w += step_tiles(b.job[j]) - 1;
   }
-  b.wg_start[b.njobs] = wg;
-  return wg;
+  b.first[b.njobs] = w;
+  b.crow = (b.njobs + XCDS - 1) / XCDS;
+  return XCDS * (b.crow + (w + XCDS - 1) / XCDS);
 }
 
 // Four consecutive elements from LDS to global memory as 16-byte
@@ -725,8 +732,12 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     jb = blockIdx.x;
     local = critical_tile(batch.job[jb]);
   } else {
-    while (jb + 1 < batch.njobs && (int)blockIdx.x >= batch.wg_start[jb + 1]) ++jb;
-    local = step_tile_of(batch.job[jb], blockIdx.x - batch.wg_start[jb]);
+    const int b = blockIdx.x, x = b % XCDS, i = b / XCDS - batch.crow, w = batch.first[batch.njobs];
+    if (i < 0) return;  // idle slot beside the critical workgroups
+    const int g = (x * w) / XCDS + i;
+    if (g >= ((x + 1) * w) / XCDS) return;
+    while (jb + 1 < batch.njobs && g >= batch.first[jb + 1]) ++jb;
+    local = step_tile_of(batch.job[jb], g - batch.first[jb]);
   }
   const FJob<T> &J = batch.job[jb];
   // the critical workgroups bound the launch: their waves issue first on
