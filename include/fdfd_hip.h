@@ -110,9 +110,10 @@ int fdfd_residual(const fdfd_grid *g, const double *op, const double *b,
 /* Replaces pardisoSolver(A, mtype=13).factor() (linalg.py:135-136): block
  * sweep factorization of T, the x-non-periodic part of A, followed by the
  * correction for the two x-wrap couplings (Woodbury: the inverse of a
- * 2ny x 2ny capacitance matrix built from corner blocks of T^-1; exact when
- * an axis has a PML of fewer than 4 cells, otherwise built from the PML-side
- * corner only, accurate to about 1e-9, refinement doing the rest). factor
+ * 2ny x 2ny capacitance matrix built from corner blocks of T^-1, exact when
+ * an axis has a PML of fewer than 4 cells; otherwise the far corners are
+ * dropped and only its order-ny Schur complement is inverted, accurate to
+ * about 1e-9, refinement doing the rest). factor
  * holds fdfd_factor_bytes, work fdfd_factor_workspace_bytes. The block
  * factorization uses one persistent launch (and waits for it) when the tile
  * grid fits the device, otherwise one launch per Gauss-Jordan step; the
