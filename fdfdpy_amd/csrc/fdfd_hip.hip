@@ -803,7 +803,7 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
       const int e = tid + u * NT, a = e / TB, b = e % TB;
       L.Pr[a][b] = pr[u];
       L.Pi[a][b] = pim[u];
-      if (mirA) {  // X[p,J][b][a] = -(D[J b'] / D[p a']) X[J,p][a][b]
+      if (mirA) {  // X[p,J][b][a] = -(D[32 J + a] / D[32 p + b]) X[J,p][a][b]
         T rr, ri;
         sym_ratio(J.dsym, J.np, tj * TB + a, p * TB + b, T(-1), rr, ri);
         L.Ar[b][a] = rr * ar[u] - ri * ai[u];
@@ -812,7 +812,7 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
         L.Ar[a][b] = ar[u];
         L.Ai[a][b] = ai[u];
       }
-      if (mirB) {  // X[I,p][b][a] = (D[p a'] / D[I b']) X[p,I][a][b]
+      if (mirB) {  // X[I,p][b][a] = (D[32 p + a] / D[32 I + b]) X[p,I][a][b]
         T rr, ri;
         sym_ratio(J.dsym, J.np, p * TB + a, ti * TB + b, T(1), rr, ri);
         L.Br[b][a] = rr * br[u] - ri * bi[u];
@@ -883,7 +883,7 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     store4_nt(dst + idx, &L.Ar[i][c0]);
     store4_nt(dst + np2 + idx, &L.Ai[i][c0]);
     if (J.dsym && p == nt - 1 && ti != tj) {
-      // G_k is symmetric under D: G_k[J,I][i][c] = (D[I c] / D[J i]) G_k[I,J][c][i]
+      // G_k is symmetric under D: G_k[J,I][i][c] = (D[32 I + c] / D[32 J + i]) G_k[I,J][c][i]
       T mr[4], mi[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
