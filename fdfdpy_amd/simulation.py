@@ -17,6 +17,7 @@ import torch
 from . import _lib
 from .constants import DEFAULT_LENGTH_SCALE, DEFAULT_SOLVER, EPSILON_0, MU_0
 from .device import DeviceOperator
+from .source import mode
 
 FIELD_NAMES = ['Ex', 'Ey', 'Ez', 'Hx', 'Hy', 'Hz']
 SOLVER_NAMES = ('hip', 'pardiso', 'scipy')
@@ -246,6 +247,17 @@ class Simulation:
         index_nl = np.sqrt(np.real(np.asarray(self.eps_r) + np.asarray(self.eps_nl)))
         index_lin = np.sqrt(np.real(np.asarray(self.eps_r)))
         return np.abs(index_nl - index_lin)
+
+    # ------------------------------------------------------------------ modes
+    def add_mode(self, neff, direction_normal, center, width, scale=1, order=1):
+        """simulation.py:50-55: a modal source definition, set up by setup_modes."""
+        self.modes.append(mode(neff, direction_normal, center, width, scale=scale, order=order))
+
+    def setup_modes(self):
+        """simulation.py:45-48: normalization run (device solve) and mode
+        profile into src, for every mode added."""
+        for m in self.modes:
+            m.setup_src(self)
 
     # ------------------------------------------------------------------ flux
     def flux_probe(self, direction_normal, center, width, nl=False):
