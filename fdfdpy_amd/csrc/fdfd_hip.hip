@@ -87,7 +87,7 @@ __host__ __device__ inline cd cinv(cd z) { cd o; crecip<double>(z.r, z.i, o.r, o
 // ------------------------------------------------------- PML (pml.py:7-39)
 // s = 1 - i sigma(l)/(omega eps0 L0), sigma = sigma_max (l/dw)^4,
 // sigma_max = -(m+1) lnR / (2 eta0 dw), m = 4, lnR = -12.
-__device__ inline cd s_of_l(double l, double dw, double omega, double L0) {
+__host__ __device__ inline cd s_of_l(double l, double dw, double omega, double L0) {
   const double eta0 = sqrt(MU_0 / EPSILON_0);
   const double sig_max = -(4.0 + 1.0) * (-12.0) / (2.0 * eta0 * dw);
   double q = l / dw;
@@ -95,7 +95,7 @@ __device__ inline cd s_of_l(double l, double dw, double omega, double L0) {
   return {1.0, -sig / (omega * EPSILON_0 * L0)};
 }
 // kind 0 = 'f', 1 = 'b'; left layer i <= npml, right layer i > nw - npml.
-__device__ inline cd sfactor(int kind, int i, int nw, int npml, double hw, double omega, double L0) {
+__host__ __device__ inline cd sfactor(int kind, int i, int nw, int npml, double hw, double omega, double L0) {
   if (npml < 1) return {1.0, 0.0};
   double dw = npml * hw;
   if (i <= npml) {
@@ -2475,6 +2475,27 @@ size_t fdfd_factor_workspace_bytes(const fdfd_grid *g, int precision) {
 size_t fdfd_solve_workspace_bytes(const fdfd_grid *g, int precision, int nrhs) {
   if (check_grid(g) || check_prec(precision) || nrhs < 1) return 0;
   return solve_work_bytes(g, precision, nrhs);
+}
+
+int fdfd_sfactors(const fdfd_grid *g, double *isxf, double *isxb, double *isyf, double *isyb) {
+  if (!g || !isxf || !isxb || !isyf || !isyb) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_sfactors");
+  if (g->nx < 1 || g->ny < 1 || g->npml_x < 0 || g->npml_y < 0 || !(g->omega > 0) || !(g->dl > 0) || !(g->L0 > 0))
+    return fail(FDFD_ERR_ARG, "fdfd_sfactors: bad grid");
+  // the same cell sizes and s-values the assembly kernel uses
+  const double hx = ((double)g->nx * g->dl) / g->nx, hy = ((double)g->ny * g->dl) / g->ny;
+  for (int i = 0; i < g->nx; ++i) {
+    const cd f = cinv(sfactor(0, i, g->nx, g->npml_x, hx, g->omega, g->L0));
+    const cd b = cinv(sfactor(1, i, g->nx, g->npml_x, hx, g->omega, g->L0));
+    isxf[2 * i] = f.r; isxf[2 * i + 1] = f.i;
+    isxb[2 * i] = b.r; isxb[2 * i + 1] = b.i;
+  }
+  for (int j = 0; j < g->ny; ++j) {
+    const cd f = cinv(sfactor(0, j, g->ny, g->npml_y, hy, g->omega, g->L0));
+    const cd b = cinv(sfactor(1, j, g->ny, g->npml_y, hy, g->omega, g->L0));
+    isyf[2 * j] = f.r; isyf[2 * j + 1] = f.i;
+    isyb[2 * j] = b.r; isyb[2 * j + 1] = b.i;
+  }
+  return FDFD_OK;
 }
 
 int fdfd_assemble(const fdfd_grid *g, const double *eps, double *op, hipStream_t stream) {

@@ -17,6 +17,7 @@ import torch
 from . import _lib
 from .constants import DEFAULT_LENGTH_SCALE, DEFAULT_SOLVER, EPSILON_0, MU_0
 from .device import DeviceOperator
+from .csr import operator_csr, stretched_derivatives
 from .source import mode
 
 FIELD_NAMES = ['Ex', 'Ey', 'Ez', 'Hx', 'Hy', 'Hz']
@@ -136,6 +137,26 @@ class Simulation:
     def operator(self):
         """The device operator (band storage + resident factors)."""
         return self._op
+
+    @property
+    def A(self):
+        """The system matrix as a host scipy CSR matrix, as fdfdpy keeps it
+        (simulation.py:82-86): copied from the device operator on first use
+        after each eps_r assignment (the linear operator; Kerr terms are
+        separate, as there)."""
+        key = (id(self._op), self._op.linear_version)
+        if getattr(self, '_A_key', None) != key:
+            self._A_csr, self._A_key = operator_csr(self._op), key
+        return self._A_csr
+
+    @property
+    def derivs(self):
+        """{'Dxf', 'Dxb', 'Dyf', 'Dyb'}: the stretched difference operators
+        of construct_A (simulation.py:87, linalg.py:55-58) as host CSR
+        matrices, built on first use."""
+        if getattr(self, '_derivs', None) is None:
+            self._derivs = stretched_derivatives(self.omega, (self.Nx, self.Ny), self.NPML, self.dl, self.L0)
+        return self._derivs
 
     # ------------------------------------------------------------------ solve
     def _src_tensor(self):

@@ -99,6 +99,14 @@ int fdfd_block_padding(void);   /* block size is padded to this multiple */
  * permittivity (linalg.py:71-72, averaging=True as the eps_r setter passes). */
 int fdfd_assemble(const fdfd_grid *g, const double *eps, double *op, hipStream_t stream);
 
+/* Replaces the s-vectors of S_create (pml.py:42-89): the reciprocal stretch
+ * factors 1/s that scale the rows of Dxf, Dxb (nx entries each) and Dyf, Dyb
+ * (ny entries each), as interleaved complex doubles in HOST arrays; the
+ * values the assembly kernel uses. For the derivative matrices of
+ * construct_A (linalg.py:55-58, 79-82) on the host; needs no GPU, and any
+ * nx, ny >= 1 (pol is not read). */
+int fdfd_sfactors(const fdfd_grid *g, double *isxf, double *isxb, double *isyf, double *isyb);
+
 /* Replaces A.dot(x) (nonlinear_solvers.py:132): y = A x for nrhs vectors. */
 int fdfd_apply(const fdfd_grid *g, const double *op, const double *x, double *y,
                int nrhs, hipStream_t stream);

@@ -297,3 +297,21 @@ def test_bounded_waits_fall_back_when_forced():
         lib.fdfd_set_spin_cap(0)
     assert rep.converged
     assert relerr(x.cpu().numpy(), xref) < 1e-10
+
+
+@pytest.mark.parametrize("shape,npml,pol", [((50, 70), [9, 9], "Ez"), ((40, 90), [8, 8], "Hz"),
+                                            ((64, 48), [0, 8], "Ez")])
+def test_host_csr_export_matches_oracle(shape, npml, pol):
+    """Simulation.A (copied from the device bands, in the caller's
+    orientation also when the solver runs transposed) and Simulation.derivs
+    against the oracle's construct_A, and A again after eps_r changes."""
+    rng = np.random.default_rng(21)
+    eps = 1 + 11.25 * (rng.random(shape) < 0.5)
+    sim = Simulation(OMEGA, eps, 0.02, npml, pol)
+    ref = OracleSimulation(OMEGA, eps, 0.02, npml, pol)
+    assert abs(sim.A - ref.A).max() <= 1e-13 * abs(ref.A).max()
+    for k in ('Dxf', 'Dxb', 'Dyf', 'Dyb'):
+        assert abs(sim.derivs[k] - ref.derivs[k]).max() <= 1e-14 * abs(ref.derivs[k]).max()
+    sim.eps_r = eps * 1.5
+    ref2 = OracleSimulation(OMEGA, eps * 1.5, 0.02, npml, pol)
+    assert abs(sim.A - ref2.A).max() <= 1e-13 * abs(ref2.A).max()
