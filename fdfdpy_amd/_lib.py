@@ -60,6 +60,8 @@ SIGNATURES = {
     "fdfd_solve_workspace_bytes": (c_size_t, [_G, c_int, c_int]),
     "fdfd_assemble": (c_int, [_G, c_void_p, c_void_p, c_void_p]),
     "fdfd_sfactors": (c_int, [_G, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fdfd_flux": (c_int, [c_int, c_int, c_double, c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p,
+                          c_void_p]),
     "fdfd_apply": (c_int, [_G, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "fdfd_residual": (c_int, [_G, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "fdfd_factor": (c_int, [_G, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),

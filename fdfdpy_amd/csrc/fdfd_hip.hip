@@ -2453,6 +2453,38 @@ void engine_loop(fdfd_engine *E) {
 
 }  // namespace
 
+// ------------------------------------------ flux probe (simulation.py:260-315)
+// out = dl * sum over the line of the time-averaged Poynting flux. kind 0:
+// Ez, x-normal (-Re(avg_x Ez conj Hy) / 2); 1: Ez, y-normal (Re(avg_y Ez
+// conj Hx) / 2); 2: Hz, x-normal (Re(Ey conj avg_x Hz) / 2); 3: Hz, y-normal
+// (-Re(Ex conj avg_y Hz) / 2). avg_x f at (i, j) is (f(i, j) + f(i+1, j)) / 2,
+// avg_y likewise along j. One workgroup, fixed-order reduction.
+__global__ void __launch_bounds__(NT) k_flux(int ny, double dl, int kind, const double *__restrict__ e,
+                                             const double *__restrict__ h, int i0, int j0, int len,
+                                             double *__restrict__ out) {
+  __shared__ double part[NT];
+  const bool xn = kind == 0 || kind == 2;
+  const double *avg_src = kind < 2 ? e : h;   // the field averaged across the line
+  const double *other = kind < 2 ? h : e;
+  double acc = 0;
+  for (int t = threadIdx.x; t < len; t += NT) {
+    const int i = xn ? i0 : i0 + t, j = xn ? j0 + t : j0;
+    const size_t p = (size_t)i * ny + j, q = xn ? p + ny : p + 1;
+    const cd a = cscale(cadd(ld_c(avg_src, p), ld_c(avg_src, q)), 0.5);
+    const cd o = ld_c(other, p);
+    // Re(u conj v) = ur vr + ui vi; the field order only changes the sign
+    const double re = a.r * o.r + a.i * o.i;
+    acc += (kind == 0 || kind == 3) ? -0.5 * re : 0.5 * re;
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) part[threadIdx.x] += part[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = dl * part[0];
+}
+
 // =================================================================== C ABI
 extern "C" {
 
@@ -2502,6 +2534,22 @@ int fdfd_assemble(const fdfd_grid *g, const double *eps, double *op, hipStream_t
   if (int rc = check_grid(g)) return rc;
   if (!eps || !op) return fail(FDFD_ERR_ARG, "eps and op must be device pointers");
   hipLaunchKernelGGL(k_assemble, dim3((g->ny + NT - 1) / NT, g->nx), dim3(NT), 0, stream, *g, eps, op);
+  HIP_TRY(hipGetLastError());
+  return FDFD_OK;
+}
+
+int fdfd_flux(int nx, int ny, double dl, int kind, const double *e, const double *h, int i0, int j0, int len,
+              double *out, hipStream_t stream) {
+  if (!e || !h || !out || kind < 0 || kind > 3 || nx < 1 || ny < 1 || len < 0)
+    return fail(FDFD_ERR_ARG, "bad arguments to fdfd_flux");
+  const bool xn = kind == 0 || kind == 2;
+  // every cell read lies inside the grid: x-normal lines read rows i0, i0+1
+  // at columns [j0, j0+len); y-normal lines rows [i0, i0+len) at columns j0, j0+1
+  const bool inside = xn ? (i0 >= 0 && i0 + 1 < nx && j0 >= 0 && j0 + len <= ny)
+                         : (j0 >= 0 && j0 + 1 < ny && i0 >= 0 && i0 + len <= nx);
+  if (!inside)
+    return fail(FDFD_ERR_ARG, "fdfd_flux: the probe line and its averaging neighbours must lie inside the grid");
+  hipLaunchKernelGGL(k_flux, dim3(1), dim3(NT), 0, stream, ny, dl, kind, e, h, i0, j0, len, out);
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
 }

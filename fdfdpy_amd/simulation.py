@@ -292,6 +292,9 @@ class Simulation:
         else:
             raise ValueError("The value of direction_normal is neither x nor y!")
         F = self.fields_nl if nl else self.fields
+        flux = self._flux_on_device(F, direction_normal, ix, iy)
+        if flux is not None:
+            return flux
         get = lambda k: F[k].cpu().numpy() if isinstance(F[k], torch.Tensor) else F[k]
         win = (slice(ix[0], ix[1] + 1), slice(iy[0], iy[1] + 1))
         cut = (slice(ix[0], ix[1]), slice(iy[0], iy[1]))
@@ -308,6 +311,32 @@ class Simulation:
         if direction_normal == "x":
             return self.dl * np.sum(0.5 * np.real(get('Ey')[cut] * np.conj(avg(Hz, 0))))
         return self.dl * np.sum(-0.5 * np.real(get('Ex')[cut] * np.conj(avg(Hz, 1))))
+
+
+    def _flux_on_device(self, F, direction_normal, ix, iy):
+        """The flux sum by the library (fdfd_flux) when the fields it needs
+        are device tensors and numpy's slicing of the probe window would not
+        be clipped by the grid edge; None otherwise (host arrays below)."""
+        kind = {('Ez', 'x'): 0, ('Ez', 'y'): 1, ('Hz', 'x'): 2, ('Hz', 'y'): 3}[(self.pol, direction_normal)]
+        e_name, h_name = [('Ez', 'Hy'), ('Ez', 'Hx'), ('Ey', 'Hz'), ('Ex', 'Hz')][kind]
+        e, h = F[e_name], F[h_name]
+        if not (isinstance(e, torch.Tensor) and isinstance(h, torch.Tensor) and e.is_cuda and h.is_cuda):
+            return None
+        if direction_normal == 'x':
+            i0, j0, n = ix[0], iy[0], iy[1] - iy[0]
+            inside = i0 >= 0 and i0 + 2 <= self.Nx and j0 >= 0 and iy[1] + 1 <= self.Ny
+        else:
+            i0, j0, n = ix[0], iy[0], ix[1] - ix[0]
+            inside = j0 >= 0 and j0 + 2 <= self.Ny and i0 >= 0 and ix[1] + 1 <= self.Nx
+        if not inside or n < 0:
+            return None
+        e = e.to(torch.complex128).contiguous()
+        h = h.to(torch.complex128).contiguous()
+        out = torch.empty(1, dtype=torch.float64, device=e.device)
+        lib = _lib.lib()
+        _lib.check(lib.fdfd_flux(self.Nx, self.Ny, self.dl, kind, e.data_ptr(), h.data_ptr(), int(i0), int(j0),
+                                 int(n), out.data_ptr(), torch.cuda.current_stream(e.device).cuda_stream))
+        return float(out.item())
 
 
 __all__ = ['Simulation', 'InputError', 'EPSILON_0', 'MU_0']

@@ -107,6 +107,18 @@ int fdfd_assemble(const fdfd_grid *g, const double *eps, double *op, hipStream_t
  * nx, ny >= 1 (pol is not read). */
 int fdfd_sfactors(const fdfd_grid *g, double *isxf, double *isxb, double *isyf, double *isyb);
 
+/* Replaces the flux sum of Simulation.flux_probe (simulation.py:260-315)
+ * for fields held on the device: out[0] (device) = dl * sum over a probe
+ * line of the time-averaged Poynting flux. Fields are complex [nx][ny] in the
+ * caller's orientation. kind 0: Ez, normal x (e = Ez, h = Hy); 1: Ez, normal
+ * y (e = Ez, h = Hx); 2: Hz, normal x (e = Ey, h = Hz); 3: Hz, normal y
+ * (e = Ex, h = Hz). The line starts at cell (i0, j0) and runs len cells along
+ * y (normal x) or x (normal y); the field averaged across it is read at the
+ * next row (normal x) or column (normal y) as well, and all of it must lie
+ * inside the grid (the caller handles lines that leave it). */
+int fdfd_flux(int nx, int ny, double dl, int kind, const double *e, const double *h, int i0, int j0,
+              int len, double *out, hipStream_t stream);
+
 /* Replaces A.dot(x) (nonlinear_solvers.py:132): y = A x for nrhs vectors. */
 int fdfd_apply(const fdfd_grid *g, const double *op, const double *x, double *y,
                int nrhs, hipStream_t stream);

@@ -315,3 +315,34 @@ def test_host_csr_export_matches_oracle(shape, npml, pol):
     sim.eps_r = eps * 1.5
     ref2 = OracleSimulation(OMEGA, eps * 1.5, 0.02, npml, pol)
     assert abs(sim.A - ref2.A).max() <= 1e-13 * abs(ref2.A).max()
+
+
+@pytest.mark.parametrize("pol", ["Ez", "Hz"])
+def test_flux_probe_on_device_matches_host_and_oracle(pol):
+    """flux_probe through fdfd_flux (device-resident fields) against the host
+    sum and the oracle's flux_probe (simulation.py:260-315); a line whose
+    window leaves the grid takes the host sum, as numpy slices it."""
+    rng = np.random.default_rng(31)
+    N = (60, 50)
+    eps = 1 + 11.25 * (rng.random(N) < 0.3)
+    src = np.zeros(N)
+    src[30, 20] = 1
+    sim_t = Simulation(OMEGA, eps, 0.02, [10, 10], pol)
+    sim_t.src = torch.tensor(src, device=DEV)
+    sim_t.solve_fields()
+    ref = OracleSimulation(OMEGA, eps, 0.02, [10, 10], pol)
+    ref.src = src
+    ref.solve_fields()
+    for direction, center, width in [('x', [35, 25], 20), ('y', [28, 24], 16), ('x', [12, 30], 13), ('y', [40, 35], 9)]:
+        F = sim_t.fields
+        ix, iy = (([center[0], center[0] + 1], [int(center[1] - width / 2), int(center[1] + width / 2)])
+                  if direction == 'x' else
+                  ([int(center[0] - width / 2), int(center[0] + width / 2)], [center[1], center[1] + 1]))
+        dev = sim_t._flux_on_device(F, direction, ix, iy)
+        assert dev is not None
+        want = ref.flux_probe(direction, center, width)
+        assert abs(dev - want) <= 1e-9 * abs(want), (direction, center, dev, want)
+        assert abs(sim_t.flux_probe(direction, center, width) - want) <= 1e-9 * abs(want)
+    # window clipped by the grid edge: host sum with numpy's slicing
+    assert sim_t._flux_on_device(sim_t.fields, 'x', [59, 60], [20, 30]) is None
+    assert sim_t.flux_probe('x', [59, 25], 10) == ref.flux_probe('x', [59, 25], 10)
