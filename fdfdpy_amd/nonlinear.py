@@ -1,11 +1,12 @@
-"""Kerr nonlinearity and the Born iteration on the device.
+"""Kerr nonlinearity, the Born iteration and Newton's method on the device.
 
 Mirrors fdfdpy/nonlinearity.py:6-37 (the Kerr model), Simulation.compute_nl
-(simulation.py:57-68) and born_solve (nonlinear_solvers.py:13-54). Each Born
-step rewrites the operator's diagonal on the device (fdfd_kerr_update) and
-re-solves; the resident factor of the linear operator serves as the
-refinement preconditioner, and the operator is refactored only when that
-refinement stops contracting.
+(simulation.py:57-68), born_solve (nonlinear_solvers.py:13-54) and
+newton_solve (nonlinear_solvers.py:57-149). Each step rewrites the
+operator's diagonal on the device (fdfd_kerr_update) and solves with it; the
+resident factor of the linear operator serves as the refinement
+preconditioner, and the operator is refactored only when that refinement
+stops contracting.
 """
 
 from __future__ import annotations
@@ -124,3 +125,84 @@ def born_solve(sim, Estart=None, conv_threshold=1e-10, max_num_iter=50, averagin
         print("the simulation did not converge, reached {}".format(convergence))
     Hx, Hy = op.fields(E)
     return sim._out(Hx), sim._out(Hy), sim._out(E), conv_array
+
+
+NEWTON_INNER_TOL = 1e-11
+NEWTON_MAX_INNER = 50
+
+
+def _conj_coupled_solve(op, j12, f, precision, tol):
+    """d with J11 d + diag(j12) conj(d) = f, J11 the operator as set: J11^-1
+    on the device (resident factor + refinement) inside a refinement of the
+    conjugate-coupled system to tol relative to ||f||. This is the system
+    solver_complex2real (linalg.py:152-186) writes in real form; the outer
+    steps contract by ||J11^-1 diag(j12)||, the Kerr term's weight."""
+    nf = torch.linalg.vector_norm(f).item()
+    d = torch.zeros_like(f)
+    if nf == 0:
+        return d
+    r = f
+    for _ in range(NEWTON_MAX_INNER):
+        dd, rep = op.solve(r, precision=precision, tol=max(tol, 1e-12))
+        d = d + dd
+        r = f - op.apply(d) - j12 * torch.conj(d)
+        if torch.linalg.vector_norm(r).item() <= tol * nf:
+            break
+    return d
+
+
+def newton_solve(sim, Estart=None, conv_threshold=1e-10, max_num_iter=50, averaging=True,
+                 precision='fp64', tol=1e-12):
+    """Newton's method for (A + Anl(E)) E = b (nonlinear_solvers.py:57-149).
+
+    With eps_nl = c |E|^2 (c real), the Jacobian of f(E) = (A + Anl(E)) E - b
+    (nl_eq_and_jac, nonlinear_solvers.py:118-149) is J11 d + J12 conj(d) with
+    J11 = A + diag(omega^2 eps0' 2 c |E|^2) -- the operator with the Kerr
+    term doubled, set on the device by fdfd_kerr_update -- and J12 =
+    diag(omega^2 eps0' c E^2). Each step solves J d = f and takes E - d;
+    the run ends with a solve of (A + Anl(E)) X = b for the fields, as the
+    reference does.
+    """
+    from .constants import EPSILON_0
+    conv_array = np.zeros((max_num_iter, 1))
+    op = sim.operator
+    if Estart is None:
+        if sim.fields['Ez'] is None:
+            (_, _, Ez) = sim.solve_fields(precision='fp64' if precision == 'fp64' else 'mixed')
+        else:
+            Ez = sim.fields['Ez']
+    else:
+        Ez = Estart
+    E = torch.as_tensor(Ez, device=sim.device).to(torch.complex128).clone()
+    coeff = kerr_coefficient(sim)
+    w = sim.omega ** 2 * EPSILON_0 * sim.L0
+    b = sim._src_tensor() * (1j * sim.omega)
+    convergence = np.inf
+    enl = None
+    try:
+        for istep in range(max_num_iter):
+            Eprev = E
+            op.kerr_update(Eprev, coeff)
+            f = op.apply(Eprev) - b
+            j12 = (w * coeff).to(torch.complex128) * Eprev * Eprev
+            op.kerr_update(Eprev, 2 * coeff)
+            E = Eprev - _conj_coupled_solve(op, j12, f, precision, NEWTON_INNER_TOL)
+            nE = torch.linalg.vector_norm(E).item()
+            convergence = torch.linalg.vector_norm(E - Eprev).item() / nE if nE > 0 else 0.0
+            conv_array[istep] = convergence
+            if convergence < conv_threshold:
+                break
+        enl = op.kerr_update(E, coeff)
+        if bool(torch.any(b != 0)):
+            X, rep = op.solve(b, precision=precision, tol=tol)
+            sim.last_report = rep
+        else:
+            X = torch.zeros_like(b)
+    finally:
+        if enl is not None:
+            sim.eps_nl = enl.cpu().numpy()
+        op.restore_linear()
+    if convergence > conv_threshold:
+        print("the simulation did not converge, reached {}".format(convergence))
+    Hx, Hy = op.fields(X)
+    return sim._out(Hx), sim._out(Hy), sim._out(X), conv_array

@@ -235,15 +235,12 @@ class Simulation:
         from .nonlinear import compute_nl
         return compute_nl(self, e)
 
-    def solve_fields_nl(self, timing=False, averaging=True, Estart=None, solver_nl='born',
+    def solve_fields_nl(self, timing=False, averaging=True, Estart=None, solver_nl='newton',
                         conv_threshold=1e-10, max_num_iter=50, solver=DEFAULT_SOLVER,
                         matrix_format='csr', precision='fp64'):
-        """Nonlinear solve (simulation.py:180-247); Born iteration on the device.
-
-        The reference's default solver_nl='newton' is not built here; 'born'
-        is the default of this implementation.
-        """
-        from .nonlinear import born_solve
+        """Nonlinear solve (simulation.py:180-247) on the device: Newton's
+        method (the reference's default) or the Born iteration."""
+        from .nonlinear import born_solve, newton_solve
         if self.pol != 'Ez':
             if self.pol == 'Hz':
                 raise ValueError('the Kerr nonlinearity is supported for pol=Ez only '
@@ -252,8 +249,12 @@ class Simulation:
         if solver_nl == 'born':
             Hx, Hy, Ez, conv_array = born_solve(self, Estart, conv_threshold, max_num_iter,
                                                 averaging=averaging, precision=PRECISIONS[precision])
-        elif solver_nl in ('newton', 'LM'):
-            raise NotImplementedError("solver_nl='{}' is not available; use solver_nl='born'".format(solver_nl))
+        elif solver_nl == 'newton':
+            Hx, Hy, Ez, conv_array = newton_solve(self, Estart, conv_threshold, max_num_iter,
+                                                  averaging=averaging, precision=PRECISIONS[precision])
+        elif solver_nl == 'LM':
+            # simulation.py:199 calls an LM_solve that the reference never defines
+            raise NotImplementedError("solver_nl='LM' is not defined by the reference either")
         else:
             raise AssertionError("solver must be one of {'born', 'newton', 'LM'}")
         self.fields_nl['Hx'] = Hx

@@ -30,6 +30,10 @@ tests live in ``tests/test_oracle_*.py``):
 * born_solve           -- chi = 0 reduces to the linear solve; the returned
                           field is a fixed point of (A + Anl(E)) E = b.
 * adjoint_gradient     -- central finite differences of the objective.
+* solver_complex2real  -- its solution satisfies A11 x + A12 conj(x) = b;
+                          dropping the conjugate coupling fails that.
+* newton_solve         -- reaches Born's fixed point (A + Anl(E)) E = b, in
+                          fewer steps, with a quadratic tail.
 """
 
 from __future__ import annotations
@@ -43,7 +47,7 @@ __all__ = [
     "EPSILON_0", "MU_0", "C_0", "ETA_0", "DEFAULT_LENGTH_SCALE",
     "sig_w", "s_value", "create_sfactor", "sfactor_vectors", "S_create",
     "createDws", "grid_average", "dL", "construct_A", "solver_direct",
-    "stretch_symmetrizer", "OracleSimulation", "OracleMode",
+    "stretch_symmetrizer", "solver_complex2real", "OracleSimulation", "OracleMode",
     "adjoint_gradient", "plane_wave_eigenvalue",
 ]
 
@@ -213,6 +217,23 @@ def solver_direct(A, b, solver='scipy'):
     return spl.spsolve(A.tocsc(), b)
 
 
+def solver_complex2real(A11, A12, b):
+    """x with A11 x + A12 conj(x) = b, as the real system of order 2N
+    [[Re A11 + Re A12, -Im A11 + Im A12], [Im A11 + Im A12, Re A11 - Re A12]]
+    [Re x; Im x] = [Re b; Im b] (linalg.py:152-186). The reference solves it
+    with Pardiso (mtype=11); its scipy branch names spsolve and zeros without
+    importing them, so this restates the evident intent with SuperLU."""
+    b = np.asarray(b).astype(np.complex128).reshape(-1)
+    N = b.size
+    if not b.any():
+        return np.zeros(b.shape, dtype=np.complex128)
+    A11, A12 = sp.csr_matrix(A11), sp.csr_matrix(A12)
+    Areal = sp.vstack((sp.hstack((A11.real + A12.real, -A11.imag + A12.imag)),
+                       sp.hstack((A11.imag + A12.imag, A11.real - A12.real)))).tocsc()
+    x = spl.spsolve(Areal, np.concatenate((b.real, b.imag)))
+    return x[:N] + 1j * x[N:]
+
+
 def stretch_symmetrizer(omega, L0, Nx, Ny, NPML, dl):
     """D = diag(sxf_i * syf_j) with D @ A complex-symmetric for both pols.
 
@@ -263,6 +284,7 @@ class OracleSimulation:
         self.eps_r = eps_r
         self.modes = []
         self.nonlinearity = []
+        self.nonlinearity_de = []
         self.eps_nl = np.zeros(eps_r.shape)
 
     @property
@@ -318,15 +340,21 @@ class OracleSimulation:
             if eps_max is None:
                 raise AssertionError("Must provide eps_max when eps_scale is True")
             f = lambda e, eps: 3 * chi_i * nl_region * np.square(np.abs(e)) * ((eps - 1) / (eps_max - 1))
+            fde = lambda e, eps: 3 * chi_i * nl_region * np.conj(e) * ((eps - 1) / (eps_max - 1))
         else:
             f = lambda e, eps: 3 * chi_i * nl_region * np.square(np.abs(e))
+            fde = lambda e, eps: 3 * chi_i * nl_region * np.conj(e)
         self.nonlinearity.append(f)
+        self.nonlinearity_de.append(fde)
 
-    # simulation.py:57-68
+    # simulation.py:57-68 (eps_nl and dnl_de)
     def compute_nl(self, e):
         self.eps_nl = np.zeros(self.eps_r.shape)
+        self.dnl_de = np.zeros(self.eps_r.shape, dtype=np.complex128)
         for f in self.nonlinearity:
             self.eps_nl = self.eps_nl + f(e, self.eps_r)
+        for fde in self.nonlinearity_de:
+            self.dnl_de = self.dnl_de + fde(e, self.eps_r)
         M = self.Nx * self.Ny
         self.Anl = sp.diags(self.omega ** 2 * EPSILON_0 * self.L0 * self.eps_nl.reshape(-1),
                             0, shape=(M, M), format='csr')
@@ -351,6 +379,41 @@ class OracleSimulation:
             conv_array[istep] = convergence
             if convergence < conv_threshold:
                 break
+        self.fields_nl.update(Hx=Hx, Hy=Hy, Ez=Ez)
+        return Hx, Hy, Ez, conv_array
+
+    # nonlinear_solvers.py:57-149 (Ez branch) with nl_eq_and_jac (:118-149):
+    # J11 = A + Anl + diag(dAde E), J12 = diag(conj(dAde) E), dAde =
+    # omega^2 eps0' dnl_de, and E <- E - Ediff with [J11 J12; conj] Ediff = f
+    def newton_solve(self, Estart=None, conv_threshold=1e-10, max_num_iter=50):
+        conv_array = np.zeros((max_num_iter, 1))
+        if self.pol != 'Ez':
+            raise ValueError('Invalid polarization: {}'.format(self.pol))
+        if Estart is None:
+            if self.fields['Ez'] is None:
+                _, _, Ez = self.solve_fields()
+            else:
+                Ez = copy.deepcopy(self.fields['Ez'])
+        else:
+            Ez = Estart
+        M = self.Nx * self.Ny
+        for istep in range(max_num_iter):
+            Eprev = Ez
+            self.compute_nl(Eprev)
+            Anl = self.A + self.Anl
+            e = Eprev.reshape(-1)
+            fE = Anl.dot(e) - self.src.reshape(-1) * 1j * self.omega
+            dAde = self.dnl_de.reshape(-1) * self.omega ** 2 * EPSILON_0 * self.L0
+            J11 = Anl + sp.diags(dAde * e, 0, shape=(M, M), format='csr')
+            J12 = sp.diags(np.conj(dAde) * e, 0, shape=(M, M), format='csr')
+            Ediff = solver_complex2real(J11, J12, fE)
+            Ez = Eprev - Ediff.reshape(self.Nx, self.Ny)
+            convergence = np.linalg.norm(Ez - Eprev) / np.linalg.norm(Ez)
+            conv_array[istep] = convergence
+            if convergence < conv_threshold:
+                break
+        self.compute_nl(Ez)
+        Hx, Hy, Ez = self.solve_fields(include_nl=True)
         self.fields_nl.update(Hx=Hx, Hy=Hy, Ez=Ez)
         return Hx, Hy, Ez, conv_array
 

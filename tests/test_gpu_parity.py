@@ -184,6 +184,37 @@ def test_born_matches_golden_trajectory():
     assert relerr(Elin, rElin) <= 1e-10
 
 
+@pytest.mark.parametrize("amp", [40.0, 80.0])
+def test_newton_matches_oracle_newton(amp):
+    """Newton's method on the device (J11 = A with the Kerr term doubled,
+    the conjugate coupling J12 refined around it) against the oracle's
+    newton_solve (its real 2N system by SuperLU): same step count, the same
+    per-step changes until round-off, the same fields, and the default
+    solver_nl of solve_fields_nl, as in the reference."""
+    N = (64, 64)
+    eps = np.ones(N)
+    eps[24:40, 24:40] = 12.25
+    region = np.zeros(N)
+    region[24:40, 24:40] = 1
+    sim = Simulation(OMEGA, eps, 0.02, [10, 10], 'Ez')
+    sim.src[12, 24:40] = amp
+    sim.add_nl(2.8e-18, region)
+    Hx, Hy, Ez, conv = sim.solve_fields_nl(conv_threshold=1e-10)
+    ref = OracleSimulation(OMEGA, eps, 0.02, [10, 10], 'Ez')
+    ref.src[12, 24:40] = amp
+    ref.add_nl(2.8e-18, region)
+    rHx, rHy, rEz, rconv = ref.newton_solve(conv_threshold=1e-10)
+    n = int(np.count_nonzero(rconv))
+    assert np.count_nonzero(conv) == n and n <= 5
+    np.testing.assert_allclose(conv.ravel()[:n - 1], rconv.ravel()[:n - 1], rtol=1e-4)
+    assert relerr(Ez, rEz) <= 1e-10
+    assert relerr(Hx, rHx) <= 1e-10 and relerr(Hy, rHy) <= 1e-10
+    np.testing.assert_allclose(sim.eps_nl, ref.eps_nl, rtol=1e-9, atol=1e-14)
+    # the same fixed point as Born
+    _, _, Eb, _ = sim.solve_fields_nl(solver_nl='born', conv_threshold=1e-12)
+    assert relerr(Ez, Eb) <= 1e-9
+
+
 def test_adjoint_gradient_matches_oracle():
     from fdfdpy_amd.adjoint import FieldSolver
     rng = np.random.default_rng(2)

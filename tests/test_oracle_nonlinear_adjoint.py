@@ -100,3 +100,41 @@ def test_kerr_diagonal_is_omega2_eps0():
     d = sim.Anl.diagonal().reshape(64, 64)
     assert abs(d[30, 30] - OMEGA ** 2 * EPSILON_0 * 1e-6 * enl) < 1e-9 * abs(d[30, 30])
     assert isinstance(sim.Anl, sp.csr_matrix)
+
+
+def test_complex2real_solves_the_conjugate_coupled_system():
+    """solver_complex2real (linalg.py:152-186) returns x with
+    A11 x + A12 conj(x) = b; dropping the conjugate coupling, or coupling
+    x instead of conj(x), fails that equation."""
+    from oracle import solver_complex2real
+    rng = np.random.default_rng(4)
+    n = 60
+    A11 = sp.random(n, n, density=0.1, random_state=1) + 1j * sp.random(n, n, density=0.1, random_state=2) \
+        + sp.eye(n) * (4 + 1j)
+    A12 = sp.diags(rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x = solver_complex2real(A11, A12, b)
+    assert np.linalg.norm(A11 @ x + A12 @ np.conj(x) - b) <= 1e-12 * np.linalg.norm(b)
+    x_plain = solver_direct(A11 + A12, b)
+    assert np.linalg.norm(A11 @ x_plain + A12 @ np.conj(x_plain) - b) > 1e-3 * np.linalg.norm(b)
+    assert not np.any(solver_complex2real(A11, A12, np.zeros(n)))
+
+
+def test_newton_reaches_the_born_fixed_point_faster():
+    """newton_solve (nonlinear_solvers.py:57-149) converges to the same
+    fixed point (A + Anl(E)) E = b as Born, in fewer steps, with the
+    quadratic tail of an exact Jacobian."""
+    born = _kerr_sim(2.8e-18)
+    _, _, Eb, conv_b = born.born_solve(conv_threshold=1e-12)
+    newton = _kerr_sim(2.8e-18)
+    _, _, En, conv_n = newton.newton_solve(conv_threshold=1e-12)
+    assert np.linalg.norm(En - Eb) <= 1e-10 * np.linalg.norm(Eb)
+    steps_b, steps_n = np.count_nonzero(conv_b), np.count_nonzero(conv_n)
+    assert steps_n < steps_b
+    c = conv_n.ravel()[:steps_n]
+    # quadratic: each change is at most ~ the square of the one before (relative)
+    assert any(c[k + 1] < 10 * c[k] ** 2 for k in range(steps_n - 1) if c[k] < 1e-2)
+    newton.compute_nl(En)
+    b = (newton.src * 1j * OMEGA).reshape(-1)
+    r = (newton.A + newton.Anl) @ En.reshape(-1) - b
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b)
