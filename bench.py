@@ -251,16 +251,26 @@ def config_timings():
         _, t = _sync_time(grad)
     out["c4_adjoint_Ez_300"] = {"objective_and_gradient_s": round(t, 4),
                                 "note": "assembly, factor, forward solve, adjoint solve, gradient kernel"}
-    # 5: Ez 400x400 Kerr Born iteration, 8 amplitudes, to relative change 1e-10
+    # 5: Ez 400x400 Kerr, 8 amplitudes, to relative change 1e-10: the Born
+    # iteration and Newton's method (the reference's default), each from the
+    # linear field of a fresh simulation
     p = C.c5_kerr()
     per_amp = []
     for a in sorted(p.extra["amps"]):
-        sim = Simulation(p.omega, p.eps, p.dl, p.npml, "Ez")
-        sim.add_nl(p.extra["chi3"], p.extra["nl_region"])
-        sim.src = p.srcs[0] * a
-        (_, _, _, conv), t = _sync_time(lambda: sim.solve_fields_nl(solver_nl="born", conv_threshold=1e-10,
-                                                                      max_num_iter=50))
-        per_amp.append({"amp": round(float(a), 3), "born_steps": int(np.count_nonzero(conv)), "s": round(t, 4)})
+        row = {"amp": round(float(a), 3)}
+        fields = {}
+        for method in ("born", "newton"):
+            sim = Simulation(p.omega, p.eps, p.dl, p.npml, "Ez")
+            sim.add_nl(p.extra["chi3"], p.extra["nl_region"])
+            sim.src = p.srcs[0] * a
+            (_, _, ez, conv), t = _sync_time(lambda: sim.solve_fields_nl(solver_nl=method, conv_threshold=1e-10,
+                                                                           max_num_iter=50))
+            fields[method] = ez
+            row[method + "_steps"] = int(np.count_nonzero(conv))
+            row[method + "_s"] = round(t, 4)
+        row["newton_vs_born_max_rel_diff"] = float(np.linalg.norm(fields["newton"] - fields["born"]) /
+                                                   np.linalg.norm(fields["born"]))
+        per_amp.append(row)
     out["c5_kerr_Ez_400"] = per_amp
     return out
 

@@ -159,9 +159,9 @@ def newton_solve(sim, Estart=None, conv_threshold=1e-10, max_num_iter=50, averag
     (nl_eq_and_jac, nonlinear_solvers.py:118-149) is J11 d + J12 conj(d) with
     J11 = A + diag(omega^2 eps0' 2 c |E|^2) -- the operator with the Kerr
     term doubled, set on the device by fdfd_kerr_update -- and J12 =
-    diag(omega^2 eps0' c E^2). Each step solves J d = f and takes E - d;
-    the run ends with a solve of (A + Anl(E)) X = b for the fields, as the
-    reference does.
+    diag(omega^2 eps0' c E^2). Each step factors J11, solves J d = f and
+    takes E - d; the run ends with a solve of (A + Anl(E)) X = b for the
+    fields, as the reference does.
     """
     from .constants import EPSILON_0
     conv_array = np.zeros((max_num_iter, 1))
@@ -186,6 +186,9 @@ def newton_solve(sim, Estart=None, conv_threshold=1e-10, max_num_iter=50, averag
             f = op.apply(Eprev) - b
             j12 = (w * coeff).to(torch.complex128) * Eprev * Eprev
             op.kerr_update(Eprev, 2 * coeff)
+            # one factorization of J11 per step (the reference refactors its
+            # Jacobian every step too); the inner solves then refine in ~2 sweeps
+            op.factor(precision, refresh=True)
             E = Eprev - _conj_coupled_solve(op, j12, f, precision, NEWTON_INNER_TOL)
             nE = torch.linalg.vector_norm(E).item()
             convergence = torch.linalg.vector_norm(E - Eprev).item() / nE if nE > 0 else 0.0
@@ -202,6 +205,8 @@ def newton_solve(sim, Estart=None, conv_threshold=1e-10, max_num_iter=50, averag
         if enl is not None:
             sim.eps_nl = enl.cpu().numpy()
         op.restore_linear()
+        # the resident factor is a Jacobian's: the next linear solve factors A afresh
+        op.drop_factor(precision)
     if convergence > conv_threshold:
         print("the simulation did not converge, reached {}".format(convergence))
     Hx, Hy = op.fields(X)
