@@ -77,3 +77,32 @@ def test_engine_factors_match_direct_factors():
         assert torch.equal(a.factors[pr][0], b.factors[pr][0])
     assert eng.launches < sum(p.shape[0] * ((p.shape[1] + 31) // 32) for p in probs)
     eng.close()
+
+
+def test_engine_batches_of_more_than_eight_jobs():
+    """12 factorizations of mixed sizes (odd and even tile counts) in shared
+    step launches: two rows of critical workgroups and tile ranges that span
+    jobs on every XCD; each equals its one-by-one launch-per-step factor."""
+    import threading
+    from fdfdpy_amd.device import DeviceOperator, FactorEngine
+    eng = FactorEngine("cuda:0")
+    sizes = (128, 160, 192, 96, 224, 128, 96, 256, 160, 96, 192, 128)
+    probs = [C.c3_request(80 + i, n=n) for i, n in enumerate(sizes)]
+    ops_a, ops_b = [], []
+    for p in probs:
+        for ops, e in ((ops_a, None), (ops_b, eng)):
+            op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, "cuda:0")
+            op.set_eps(p.eps)
+            op.engine = e
+            ops.append(op)
+    for op in ops_a:
+        op.factor("c64", stepwise=True)
+    threads = [threading.Thread(target=op.factor, args=("c64",)) for op in ops_b]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    torch.cuda.synchronize()
+    for a, b in zip(ops_a, ops_b):
+        assert torch.equal(a.factors["c64"][0], b.factors["c64"][0])
+    eng.close()
