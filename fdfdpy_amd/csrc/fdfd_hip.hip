@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <condition_variable>
 #include <thread>
@@ -234,8 +235,9 @@ __global__ void k_selftest_mfma_f32(const double *a, const double *b, double *d)
 // Every wait is bounded: after SPIN_CAP polls it raises an abort flag, and
 // every later wait returns at once, so a grid always drains.
 constexpr unsigned SPIN_CAP = 1u << 21;
-unsigned g_spin_cap = SPIN_CAP;  // host setting passed to the persistent kernels
-int g_step_sym = 1;              // launch-per-step factorizations use symmetric steps
+// host settings, read by the calling threads and the engines' scheduler threads
+std::atomic<unsigned> g_spin_cap{SPIN_CAP};  // passed to the persistent kernels
+std::atomic<int> g_step_sym{1};              // launch-per-step factorizations use symmetric steps
 
 template <typename T>
 __device__ inline T ld_xcd(const T *p) {
@@ -2083,7 +2085,7 @@ int factor_persistent(const fdfd_grid *g, const double *op, void *factor, void *
   PJob<T> J;
   J.op = op; J.G = (T *)factor; J.xrow = (T *)fw.W; J.xcol = (T *)fw.xcol; J.pinv = (T *)fw.pinv_all;
   J.sync = fw.sync; J.status = fw.status;
-  J.m = g->nx; J.n = n; J.np = np; J.nt = nt; J.cap = g_spin_cap;
+  J.m = g->nx; J.n = n; J.np = np; J.nt = nt; J.cap = g_spin_cap.load();
   hipLaunchKernelGGL((k_gj_persist<T, PIV>), dim3(nt * nt), dim3(NT), 0, s, J);
   HIP_TRY(hipGetLastError());
   unsigned ab = 0;
@@ -2104,7 +2106,7 @@ int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, 
   b.njobs = 1;
   FJob<T> &J = b.job[0];
   J.op = op; J.G = (T *)factor; J.W = (T *)fw.W; J.Pinv = (T *)fw.Pinv; J.status = fw.status;
-  J.dsym = g_step_sym ? (T *)fw.xcol : nullptr;
+  J.dsym = g_step_sym.load() ? (T *)fw.xcol : nullptr;
   J.m = m; J.n = n; J.np = np; J.nt = nt;
   J.k = -1; J.p = 0;
   int rc = launch_step<T, PIV>(b, s);
@@ -2280,7 +2282,7 @@ void launch_psweep(int W, size_t lds, hipStream_t s, const T *G, const double *o
                               (int)(2 * RP * 768 * sizeof(T)));
   });
   hipLaunchKernelGGL((k_psweep<T, RP>), dim3(W), dim3(NT), lds, s, G, op, r, y, x, nx, ny, np, nr, dir, sy,
-                     g_spin_cap);
+                     g_spin_cap.load());
 }
 
 // x += F^-1 r with the persistent sweep kernels (RHS in passes of
@@ -2841,7 +2843,7 @@ int fdfd_engine_submit(fdfd_engine *E, const fdfd_grid *g, const double *op, voi
   e->g = *g; e->op = op; e->factor = factor; e->work = work;
   e->precision = base_prec(precision); e->pivot = precision & FDFD_FACTOR_PIVOT;
   e->m = g->nx; e->n = g->ny; e->np = pad_n(g->ny); e->nt = e->np / TB;
-  e->sym = g_step_sym != 0;
+  e->sym = g_step_sym.load() != 0;
   HIP_TRY(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
   {
     std::lock_guard<std::mutex> lk(E->mu);
@@ -2877,11 +2879,10 @@ uint64_t fdfd_engine_launches(fdfd_engine *E) {
   return E->launches;
 }
 
-void fdfd_set_spin_cap(unsigned polls) { g_spin_cap = polls ? polls : SPIN_CAP; }
+void fdfd_set_spin_cap(unsigned polls) { g_spin_cap.store(polls ? polls : SPIN_CAP); }
 
 int fdfd_set_step_symmetry(int on) {
-  const int prev = g_step_sym;
-  g_step_sym = on ? 1 : 0;
+  const int prev = g_step_sym.exchange(on ? 1 : 0);
   return prev;
 }
 
