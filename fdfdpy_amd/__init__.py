@@ -4,4 +4,8 @@
 (fdfdpy/__init__.py:3).
 """
 
+from .simulation import InputError, Simulation
+
 name = "fdfdpy_amd"
+
+__all__ = ["Simulation", "InputError", "name"]

@@ -5,9 +5,12 @@
 // size n = ny, padded to TB) of the x-non-periodic part T of A:
 //     S_0 = T_0,  S_k = T_k - diag(l_k) G_{k-1} diag(u_{k-1}),  G_k = S_k^-1
 // with l_k / u_k the couplings to the previous / next column. Each G_k is
-// formed by tiled Gauss-Jordan (TB x TB pivot tiles inverted with partial
-// pivoting inside the tile, rank-TB updates on MFMA). The periodic x-wrap of
-// A (derivatives.py:22,25) is left to refinement against the exact operator.
+// formed by tiled Gauss-Jordan (TB x TB diagonal pivot tiles, inverted with
+// 2 x 2 block pivots inside the tile, or with partial pivoting inside the
+// tile in the fallback factor; rank-TB updates on MFMA). The periodic x-wrap
+// of A (derivatives.py:22,25) is handled by a Woodbury correction formed
+// after the block factorization; refinement against the exact operator
+// removes what a truncated correction leaves.
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdint.h>
@@ -34,7 +37,8 @@ constexpr int TB = 32;        // pivot tile / output tile edge
 constexpr int TBP = TB + 1;   // padded LDS row
 constexpr int NT = 256;       // threads per workgroup (4 waves)
 constexpr int MAXJ = 16;      // jobs per batched step launch
-constexpr int MAXR = 4;       // right-hand sides per sweep pass
+constexpr int MAXR = 4;       // right-hand sides per sweep pass (per-block sweeps)
+constexpr int MAX_BLOCK_N = 2048;   // largest block (the shorter grid side)
 constexpr int NORM_PARTS = 512;
 
 // constants.py:3-6
@@ -1120,7 +1124,8 @@ template <typename T>
 __global__ void __launch_bounds__(NT) k_sweep_fwd(const T *__restrict__ G, const double *__restrict__ op,
                                                   const double *__restrict__ r, T *__restrict__ y,
                                                   int nx, int ny, int np, int k, int nrhs) {
-  __shared__ T vr[MAXR][768 + TB], vi[MAXR][768 + TB];
+  extern __shared__ unsigned char sweep_lds[];
+  T *vr = reinterpret_cast<T *>(sweep_lds), *vi = vr + (size_t)MAXR * np;   // [MAXR][np] each
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const size_t np2 = (size_t)np * np, M = (size_t)nx * ny;
   const T *Gk = G + (size_t)k * 2 * np2;
@@ -1138,8 +1143,8 @@ __global__ void __launch_bounds__(NT) k_sweep_fwd(const T *__restrict__ G, const
         }
         xr = v.r; xi = v.i;
       }
-      vr[rr][c] = (T)xr;
-      vi[rr][c] = (T)xi;
+      vr[rr * np + c] = (T)xr;
+      vi[rr * np + c] = (T)xi;
     }
     __syncthreads();
     if (a < np) {
@@ -1151,8 +1156,8 @@ __global__ void __launch_bounds__(NT) k_sweep_fwd(const T *__restrict__ G, const
 #pragma unroll
         for (int q = 0; q < MAXR; ++q) {
           if (q < nr) {
-            ar[q] += gr * vr[q][c] - gi * vi[q][c];
-            ai[q] += gr * vi[q][c] + gi * vr[q][c];
+            ar[q] += gr * vr[q * np + c] - gi * vi[q * np + c];
+            ai[q] += gr * vi[q * np + c] + gi * vr[q * np + c];
           }
         }
       }
@@ -1180,7 +1185,8 @@ template <typename T>
 __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const double *__restrict__ op,
                                                   T *__restrict__ y, double *__restrict__ x,
                                                   int nx, int ny, int np, int k, int nrhs) {
-  __shared__ T vr[MAXR][768 + TB], vi[MAXR][768 + TB];
+  extern __shared__ unsigned char sweep_lds[];
+  T *vr = reinterpret_cast<T *>(sweep_lds), *vi = vr + (size_t)MAXR * np;   // [MAXR][np] each
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const size_t np2 = (size_t)np * np, M = (size_t)nx * ny;
   const T *Gk = G + (size_t)k * 2 * np2;
@@ -1195,8 +1201,8 @@ __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const
         cd v = cmul(ld_c(op, FDFD_OP_XP * M + (size_t)k * ny + c), cd{(double)dn[0], (double)dn[1]});
         xr = v.r; xi = v.i;
       }
-      vr[rr][c] = (T)xr;
-      vi[rr][c] = (T)xi;
+      vr[rr * np + c] = (T)xr;
+      vi[rr * np + c] = (T)xi;
     }
     __syncthreads();
     if (a < np) {
@@ -1209,8 +1215,8 @@ __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const
 #pragma unroll
           for (int q = 0; q < MAXR; ++q) {
             if (q < nr) {
-              ar[q] += gr * vr[q][c] - gi * vi[q][c];
-              ai[q] += gr * vi[q][c] + gi * vr[q][c];
+              ar[q] += gr * vr[q * np + c] - gi * vi[q * np + c];
+              ai[q] += gr * vi[q * np + c] + gi * vr[q * np + c];
             }
           }
         }
@@ -1264,21 +1270,22 @@ __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const
 // cache; v rows are padded to a multiple of 64 columns with zeros, so the
 // inner loop needs no per-lane guard.
 constexpr int SRW = 2;              // rows per wave
-constexpr int MAXCL = 768 / 64;     // columns per lane at the largest block
+constexpr int PSWEEP_MAX_N = 768;   // largest padded block of the persistent sweeps
+constexpr int MAXCL = PSWEEP_MAX_N / 64;   // columns per lane at that block
 constexpr int MAXRHS = 8;           // right-hand sides per pass (complex64)
 // right-hand sides per pass: FP64 holds half as many row sums in registers
 template <typename T> constexpr int sweep_rhs() { return sizeof(T) == 8 ? MAXRHS / 2 : MAXRHS; }
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 inline int pad64(int n) { return (n + 63) / 64 * 64; }
 
-constexpr int MAXW = 768 / (4 * SRW);   // sweep workgroups at the largest block
+constexpr int MAXW = PSWEEP_MAX_N / (4 * SRW);   // sweep workgroups at that block
 
 struct SweepSync {
   unsigned abort;
   unsigned pad[63];
   unsigned done[2][MAXW];   // forward / backward: steps finished per workgroup
   // the next block's input vector, published row by row: [2 parity][RP][np] complex
-  double vbuf[2 * MAXRHS * 768];
+  double vbuf[2 * MAXRHS * PSWEEP_MAX_N];
 };
 
 // Sums v[0..N) over the 64 lanes of a wave at once: each round halves the
@@ -1376,7 +1383,7 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
     //    zero at the first backward step, else the vector all workgroups
     //    published at the previous step (every load issued before any use)
     if (s == 0) {
-      constexpr int CU = (768 + NT - 1) / NT;
+      constexpr int CU = (PSWEEP_MAX_N + NT - 1) / NT;
       double ar[RP][CU], ai[RP][CU];
 #pragma unroll
       for (int q = 0; q < RP; ++q)
@@ -1402,7 +1409,7 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
     } else {
       wait_all_flags(sy->done[dir], W, (unsigned)s, &sy->abort, cap);
       const T *vin = vb + (size_t)(s & 1) * RP * np * 2;
-      constexpr int CU = (768 + NT - 1) / NT;
+      constexpr int CU = (PSWEEP_MAX_N + NT - 1) / NT;
       T br[RP][CU], bi[RP][CU];
 #pragma unroll
       for (int q = 0; q < RP; ++q)
@@ -1943,7 +1950,9 @@ __global__ void __launch_bounds__(NT) k_fields(fdfd_grid g, const double *__rest
 int check_grid(const fdfd_grid *g) {
   if (!g) return fail(FDFD_ERR_ARG, "grid is NULL");
   if (g->nx < 2 || g->ny < 2) return fail(FDFD_ERR_ARG, "grid must be at least 2x2, got %dx%d", g->nx, g->ny);
-  if (g->ny > 768) return fail(FDFD_ERR_ARG, "block size ny=%d exceeds the supported 768 (sweep along the shorter axis)", g->ny);
+  if (g->ny > MAX_BLOCK_N)
+    return fail(FDFD_ERR_ARG, "block size ny=%d exceeds the supported %d (sweep along the shorter axis)", g->ny,
+                MAX_BLOCK_N);
   if (g->npml_x < 0 || g->npml_y < 0) return fail(FDFD_ERR_ARG, "both elements of NPML must be >= 0");
   if (g->pol != FDFD_POL_EZ && g->pol != FDFD_POL_HZ) return fail(FDFD_ERR_ARG, "pol must be Ez or Hz");
   if (!(g->omega > 0) || !(g->dl > 0) || !(g->L0 > 0)) return fail(FDFD_ERR_ARG, "omega, dl and L0 must be positive");
@@ -2126,10 +2135,19 @@ int sweep_blockwise(const fdfd_grid *g, const double *op, const void *factor, co
                     double *x, int nrhs, void *y, hipStream_t s) {
   const int nx = g->nx, ny = g->ny, np = pad_n(ny);
   const dim3 grid((np + 3) / 4);
+  const size_t lds = (size_t)2 * MAXR * np * sizeof(T);
+  static std::once_flag lds_attr;
+  std::call_once(lds_attr, [] {
+    const int most = (int)(2 * MAXR * pad_n(MAX_BLOCK_N) * sizeof(T));
+    (void)hipFuncSetAttribute((const void *)k_sweep_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, most);
+    (void)hipFuncSetAttribute((const void *)k_sweep_bwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, most);
+  });
   for (int k = 0; k < nx; ++k)
-    hipLaunchKernelGGL(k_sweep_fwd<T>, grid, dim3(NT), 0, s, (const T *)factor, op, r, (T *)y, nx, ny, np, k, nrhs);
+    hipLaunchKernelGGL(k_sweep_fwd<T>, grid, dim3(NT), lds, s, (const T *)factor, op, r, (T *)y, nx, ny, np, k,
+                       nrhs);
   for (int k = nx - 1; k >= 0; --k)
-    hipLaunchKernelGGL(k_sweep_bwd<T>, grid, dim3(NT), 0, s, (const T *)factor, op, (T *)y, x, nx, ny, np, k, nrhs);
+    hipLaunchKernelGGL(k_sweep_bwd<T>, grid, dim3(NT), lds, s, (const T *)factor, op, (T *)y, x, nx, ny, np, k,
+                       nrhs);
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
 }
@@ -2279,7 +2297,7 @@ void launch_psweep(int W, size_t lds, hipStream_t s, const T *G, const double *o
   static std::once_flag lds_attr;
   std::call_once(lds_attr, [] {
     (void)hipFuncSetAttribute((const void *)k_psweep<T, RP>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(2 * RP * 768 * sizeof(T)));
+                              (int)(2 * RP * PSWEEP_MAX_N * sizeof(T)));
   });
   hipLaunchKernelGGL((k_psweep<T, RP>), dim3(W), dim3(NT), lds, s, G, op, r, y, x, nx, ny, np, nr, dir, sy,
                      g_spin_cap.load());
@@ -2290,7 +2308,8 @@ void launch_psweep(int W, size_t lds, hipStream_t s, const T *G, const double *o
 template <typename T>
 int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const double *r,
                double *x, int nrhs, const SolveWork &sw, hipStream_t s, bool blockwise) {
-  if (blockwise) return sweep_blockwise<T>(g, op, factor, r, x, nrhs, sw.y, s);
+  // blocks beyond the persistent kernel's register tiles: one launch per block
+  if (blockwise || pad_n(g->ny) > PSWEEP_MAX_N) return sweep_blockwise<T>(g, op, factor, r, x, nrhs, sw.y, s);
   const int nx = g->nx, ny = g->ny, np = pad_n(ny);
   const int W = (np + 4 * SRW - 1) / (4 * SRW);
   const size_t M = (size_t)nx * ny;
@@ -2379,7 +2398,12 @@ void fill_job(FJob<T> &J, const EngineJob &e) {
   J.m = e.m; J.n = e.n; J.np = e.np; J.nt = e.nt; J.k = e.k; J.p = e.p;
 }
 
-// One batched launch over up to MAXJ jobs of precision T; advances their steps.
+// One batched launch over up to MAXJ jobs of precision T; advances their
+// steps. Only the scheduler thread touches a job's k and p. A job whose last
+// step is on the stream gets its completion event recorded first; then,
+// under E->mu, its rc and launched_all are set (waiters read them under the
+// same lock), so no waiter can see launched_all before the event exists on
+// the stream.
 template <typename T, bool PIV>
 int engine_launch(fdfd_engine *E, std::vector<std::shared_ptr<EngineJob>> &sel) {
   FBatch<T> b;
@@ -2388,15 +2412,19 @@ int engine_launch(fdfd_engine *E, std::vector<std::shared_ptr<EngineJob>> &sel) 
   for (int j = 0; j < b.njobs; ++j) fill_job<T>(b.job[j], *sel[j]);
   const int wg = batch_layout(b);
   hipLaunchKernelGGL((k_gj_step<T, PIV>), dim3(wg), dim3(NT), 0, E->stream, b);
-  hipError_t err = hipGetLastError();
+  const hipError_t err = hipGetLastError();
+  std::vector<std::pair<EngineJob *, int>> finished;  // job, rc
   for (auto &e : sel) {
-    if (err != hipSuccess) { e->rc = FDFD_ERR_HIP; e->launched_all = true; continue; }
+    if (err != hipSuccess) { finished.emplace_back(e.get(), FDFD_ERR_HIP); continue; }
     if (e->k < 0) { e->k = 0; e->p = 0; continue; }
     if (++e->p == e->nt) { e->p = 0; ++e->k; }
-    if (e->k == e->m) {
-      e->launched_all = true;
-      if (hipEventRecord(e->done, E->stream) != hipSuccess) e->rc = FDFD_ERR_HIP;
-    }
+    if (e->k == e->m)
+      finished.emplace_back(e.get(), hipEventRecord(e->done, E->stream) == hipSuccess ? FDFD_OK : FDFD_ERR_HIP);
+  }
+  std::lock_guard<std::mutex> lk(E->mu);
+  for (auto &f : finished) {
+    f.first->rc = f.second;
+    f.first->launched_all = true;
   }
   ++E->launches;
   return err == hipSuccess ? FDFD_OK : FDFD_ERR_HIP;
@@ -2858,19 +2886,24 @@ int fdfd_engine_submit(fdfd_engine *E, const fdfd_grid *g, const double *op, voi
 int fdfd_engine_wait(fdfd_engine *E, uint64_t ticket) {
   if (!E) return fail(FDFD_ERR_ARG, "engine is NULL");
   std::shared_ptr<EngineJob> e;
+  int rc = FDFD_OK;
   {
     std::unique_lock<std::mutex> lk(E->mu);
     auto it = E->jobs.find(ticket);
     if (it == E->jobs.end()) return fail(FDFD_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
     e = it->second;
     E->cv_done.wait(lk, [&] { return e->launched_all; });
+    rc = e->rc;
     E->jobs.erase(it);
   }
-  hipError_t err = hipEventSynchronize(e->done);
+  // rc == FDFD_OK: the event was recorded before launched_all was set
+  hipError_t err = rc == FDFD_OK ? hipEventSynchronize(e->done) : hipSuccess;
+  if (rc != FDFD_OK) (void)hipStreamSynchronize(E->stream);  // no step of the job still queued
   (void)hipEventDestroy(e->done);
+  e->done = nullptr;
   if (err != hipSuccess) return fail(FDFD_ERR_HIP, "engine factor: %s", hipGetErrorString(err));
-  if (e->rc) return fail(e->rc, "engine factor launch failed");
-  return fdfd_factor_status(&e->g, e->precision, e->work, E->stream);
+  if (rc) return fail(rc, "engine factor launch failed");
+  return FDFD_OK;
 }
 
 uint64_t fdfd_engine_launches(fdfd_engine *E) {

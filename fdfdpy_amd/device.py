@@ -23,7 +23,7 @@ import torch
 
 from . import _lib as L
 
-MAX_BLOCK = 768
+MAX_BLOCK = 2048   # the library's largest block (the shorter grid side)
 
 
 def _vp(t):
@@ -71,8 +71,10 @@ class SolveReport:
     converged: bool
     max_rel_residual: float
     rel_residual: list
-    fell_back: bool = False
+    fell_back: bool = False     # solved with a factor other than the requested one
+    pivoted: bool = False       # the factor pivots inside its 32x32 pivot tiles
     restarts: int = 0           # persistent sweeps timed out; redone per block
+    failed_factors: list = field(default_factory=list)   # (precision, pivoted) with a zero/non-finite pivot
     factor_seconds: float = 0.0
     solve_seconds: float = 0.0
     history: list = field(default_factory=list)
@@ -244,26 +246,41 @@ class DeviceOperator:
         nrhs = bb.shape[0]
         bs = self.to_solver(bb.to(device=self.device, dtype=torch.complex128))
         x = torch.empty_like(bs)
-        rep = self._solve_one(bs, x, nrhs, precision, tol, max_iter, transpose)
-        attempts = []
-        if self.factors[precision][1] != self.version:
-            # the resident factor belongs to an earlier diagonal (Born steps)
-            attempts.append((precision, False))
+        # attempts in order: (precision, pivoted, refactor first)
+        plan = [(precision, False, False)]
+        if precision in self.factors and self.factors[precision][1] != self.version:
+            # the resident factor belongs to an earlier diagonal (Born steps):
+            # it preconditions the refinement first, then is rebuilt
+            plan.append((precision, False, True))
         if precision == "c64" and fallback:
-            attempts.append(("fp64", False))
+            plan.append(("fp64", False, True))
         if fallback:
             # last resort: partial pivoting inside the 32x32 pivot tiles
-            attempts.append(("fp64", True))
-        for prec, piv in attempts:
-            if rep.converged:
+            plan.append(("fp64", True, True))
+        rep, failed, spent = None, [], [0.0, 0.0, 0]
+        for prec, piv, refresh in plan:
+            if rep is not None and rep.converged:
                 break
-            self.factor(prec, refresh=True, pivot=piv)
-            rep2 = self._solve_one(bs, x, nrhs, prec, tol, max_iter, transpose)
+            try:
+                if refresh:
+                    self.factor(prec, refresh=True, pivot=piv)
+                rep2 = self._solve_one(bs, x, nrhs, prec, tol, max_iter, transpose)
+            except L.FdfdError as e:
+                if e.code != L.FDFD_ERR_NUMERIC:
+                    raise
+                # a zero or non-finite pivot: this factor is unusable
+                self.drop_factor(prec)
+                failed.append((prec, piv))
+                continue
             rep2.fell_back = prec != precision or piv
-            rep2.factor_seconds += rep.factor_seconds
-            rep2.solve_seconds += rep.solve_seconds
-            rep2.restarts += rep.restarts
+            rep2.pivoted = self.factors[prec][2]
+            spent = [spent[0] + rep2.factor_seconds, spent[1] + rep2.solve_seconds, spent[2] + rep2.restarts]
             rep = rep2
+        if rep is None:
+            raise L.FdfdError(L.FDFD_ERR_NUMERIC, f"every factor in the fallback order had a zero or "
+                              f"non-finite pivot: {failed}")
+        rep.factor_seconds, rep.solve_seconds, rep.restarts = spent
+        rep.failed_factors = failed
         if not rep.converged:
             raise L.NotConverged(L.FDFD_ERR_NOT_CONVERGED,
                                  f"solve did not reach rel. residual {tol:g}: {rep.max_rel_residual:.3e} "
@@ -325,6 +342,12 @@ class DeviceOperator:
         L.check(self.lib.fdfd_kerr_update(ctypes.byref(self.grid), _vp(self.op), _vp(self.base_c), _vp(es),
                                           _vp(chi), _vp(enl), _stream(self.device)))
         return self.to_user(enl)
+
+    def set_kerr_diagonal(self, eps_nl):
+        """op.C = linear C + omega^2 eps0 L0 eps_nl for a given real eps_nl
+        (caller layout): A + Anl of simulation.py:121-124."""
+        e = torch.ones(self.shape, dtype=torch.complex128, device=self.device)
+        return self.kerr_update(e, eps_nl)
 
     def grad_eps(self, lam, e):
         """-Re(lam * omega^2 eps0 L0 * e) (Ez), both in caller layout."""

@@ -12,7 +12,10 @@ stops contracting.
 from __future__ import annotations
 
 import numpy as np
+import scipy.sparse as sp
 import torch
+
+from .constants import EPSILON_0
 
 
 class Nonlinearity:
@@ -45,10 +48,14 @@ class Nonlinearity:
         return self.coefficient(eps_r) * np.square(np.abs(e))
 
     def dnl_de(self, e, eps_r):
-        c = 3 * self.chi * np.asarray(self.nl_region)
-        if self.eps_scale:
-            c = c * ((np.real(np.asarray(eps_r)) - 1) / (self.eps_max - 1))
-        return c * np.conj(e)
+        return self.coefficient(eps_r) * np.conj(e)
+
+    def dnl_deps(self, e, eps_r):
+        """d eps_nl / d eps_r: 3 chi nl_region |e|^2 / (eps_max - 1) when
+        eps_scale is set, else 0 (nonlinearity.py:24, :31)."""
+        if not self.eps_scale:
+            return np.zeros(np.shape(self.nl_region))
+        return 3 * self.chi * np.asarray(self.nl_region, dtype=np.float64) * np.square(np.abs(e)) / (self.eps_max - 1)
 
 
 def kerr_coefficient(sim):
@@ -60,11 +67,36 @@ def kerr_coefficient(sim):
     return torch.as_tensor(c, dtype=torch.float64, device=sim.device)
 
 
+def store_nl_state(sim, e, enl=None):
+    """The host attributes compute_nl leaves on the simulation
+    (simulation.py:57-68): eps_nl, dnl_de, dnl_deps (arrays of the grid's
+    shape, summed over the nonlinearities) and Anl, the diagonal CSR matrix
+    omega^2 eps0 L0 eps_nl. enl, when given, is eps_nl as computed on the
+    device for the same field."""
+    eh = e.detach().cpu().numpy() if isinstance(e, torch.Tensor) else np.asarray(e)
+    eps = sim.eps_r.detach().cpu().numpy() if isinstance(sim.eps_r, torch.Tensor) else np.asarray(sim.eps_r)
+    shape = (sim.Nx, sim.Ny)
+    eps_nl = np.zeros(shape)
+    dnl_de = np.zeros(shape, dtype=np.complex128)
+    dnl_deps = np.zeros(shape)
+    for nli in sim.nonlinearity:
+        eps_nl = eps_nl + nli.eps_nl(eh, eps)
+        dnl_de = dnl_de + nli.dnl_de(eh, eps)
+        dnl_deps = dnl_deps + nli.dnl_deps(eh, eps)
+    if enl is not None:
+        eps_nl = enl.detach().cpu().numpy() if isinstance(enl, torch.Tensor) else np.asarray(enl)
+    M = sim.Nx * sim.Ny
+    sim.eps_nl = eps_nl
+    sim.dnl_de = dnl_de
+    sim.dnl_deps = dnl_deps
+    sim.Anl = sp.spdiags(sim.omega ** 2 * EPSILON_0 * sim.L0 * eps_nl.reshape((-1,)), 0, M, M, format='csr')
+
+
 def compute_nl(sim, e):
     """eps_nl of field e and the nonlinear diagonal on the device (simulation.py:57-68)."""
     et = torch.as_tensor(e, device=sim.device).to(torch.complex128)
     enl = sim.operator.kerr_update(et, kerr_coefficient(sim))
-    sim.eps_nl = enl.cpu().numpy()
+    store_nl_state(sim, et, enl)
     return enl
 
 
@@ -119,7 +151,8 @@ def born_solve(sim, Estart=None, conv_threshold=1e-10, max_num_iter=50, averagin
                 break
     finally:
         if enl is not None:
-            sim.eps_nl = enl.cpu().numpy()
+            # the state of the reference's last compute_nl(Eprev)
+            store_nl_state(sim, Eprev, enl)
         op.restore_linear()
     if convergence > conv_threshold:
         print("the simulation did not converge, reached {}".format(convergence))
@@ -163,7 +196,6 @@ def newton_solve(sim, Estart=None, conv_threshold=1e-10, max_num_iter=50, averag
     takes E - d; the run ends with a solve of (A + Anl(E)) X = b for the
     fields, as the reference does.
     """
-    from .constants import EPSILON_0
     conv_array = np.zeros((max_num_iter, 1))
     op = sim.operator
     if Estart is None:
@@ -203,7 +235,8 @@ def newton_solve(sim, Estart=None, conv_threshold=1e-10, max_num_iter=50, averag
             X = torch.zeros_like(b)
     finally:
         if enl is not None:
-            sim.eps_nl = enl.cpu().numpy()
+            # the reference's final compute_nl(Ez) before its last solve
+            store_nl_state(sim, E, enl)
         op.restore_linear()
         # the resident factor is a Jacobian's: the next linear solve factors A afresh
         op.drop_factor(precision)

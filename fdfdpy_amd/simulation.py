@@ -16,7 +16,7 @@ import torch
 
 from . import _lib
 from .constants import DEFAULT_LENGTH_SCALE, DEFAULT_SOLVER, EPSILON_0, MU_0
-from .device import DeviceOperator
+from .device import MAX_BLOCK, DeviceOperator
 from .csr import operator_csr, stretched_derivatives
 from .source import mode
 
@@ -61,6 +61,9 @@ class Simulation:
         self.Ny = Ny
         if 2 * self.NPML[0] > Nx or 2 * self.NPML[1] > Ny:
             raise InputError("NPML {} does not fit the {}x{} domain".format(self.NPML, Nx, Ny))
+        if min(Nx, Ny) > MAX_BLOCK:
+            raise InputError("grid {}x{}: the shorter side must be <= {} cells (the direct solver's "
+                             "block size)".format(Nx, Ny, MAX_BLOCK))
         self.mu_r = np.ones((self.Nx, self.Ny))
         self.src = np.zeros((self.Nx, self.Ny))
         self.xrange = [0, float(Nx * self.dl)]
@@ -74,6 +77,8 @@ class Simulation:
         self.modes = []
         self.nonlinearity = []
         self.eps_nl = np.zeros(eps_r.shape)
+        self.dnl_de = np.zeros(eps_r.shape)
+        self.dnl_deps = np.zeros(eps_r.shape)
         self.last_report = None
 
     # ------------------------------------------------------------------ checks
@@ -198,7 +203,10 @@ class Simulation:
             raise ValueError('Invalid polarization: {}'.format(str(self.pol)))
         op = self._op
         if include_nl:
+            # A + Anl (simulation.py:121-124): the diagonal of the device
+            # operator is set from sim.eps_nl, whatever it held before
             eps_tot = self._eps_total()
+            op.set_kerr_diagonal(torch.as_tensor(np.real(np.asarray(self.eps_nl)), dtype=torch.float64))
         else:
             op.restore_linear()
             eps_tot = None

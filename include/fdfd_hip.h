@@ -205,9 +205,11 @@ int fdfd_fields(const fdfd_grid *g, const double *eps, const double *x, double *
  * new ones join at the next launch. Arguments of fdfd_engine_submit are as
  * for fdfd_factor and must stay valid until fdfd_engine_wait returns; the
  * operator must be complete on the device before submission.
- * fdfd_engine_wait blocks until the block factorization is complete and
- * returns its numeric status (as fdfd_factor_status); fdfd_factor_wrap then
- * completes the factor. */
+ * fdfd_engine_wait blocks until the block factorization is complete on the
+ * device (it waits for that job's completion event only, not for other jobs'
+ * steps) and returns FDFD_OK or the launch error; fdfd_factor_wrap then
+ * completes the factor on the caller's stream, and fdfd_factor_status reads
+ * its numeric status there. */
 typedef struct fdfd_engine fdfd_engine;
 fdfd_engine *fdfd_engine_create(int device);
 int fdfd_engine_destroy(fdfd_engine *engine);

@@ -59,7 +59,35 @@ def main():
     with open(os.path.join(HERE, 'reference_values.json'), 'w') as f:
         json.dump({'notebooks/Examples.ipynb:105 W_in': 0.0016689050319357455,
                    'notebooks/Examples_nonlinear.ipynb:143': 2.7e-06}, f, indent=1)
+    c5_stated()
+
+
+def c5_stated(n_steps=4):
+    """The Kerr workload at its stated line-source amplitudes (1e8 and 1e9):
+    the largest eps_nl of the linear field and the first Born steps'
+    relative changes (nonlinear_solvers.py:13-54). Ez 400 x 400, NPML 20,
+    eps 12.25 and chi3 = 2.8e-18 on [160:240]^2, src[50, 170:230] = amp."""
+    N = 400
+    omega = 2 * np.pi * C_0 / 1.55e-6
+    eps = np.ones((N, N))
+    eps[160:240, 160:240] = 12.25
+    region = np.zeros((N, N))
+    region[160:240, 160:240] = 1
+    out = {}
+    for amp in (1e8, 1e9):
+        sim = OracleSimulation(omega, eps, 0.02, [20, 20], 'Ez')
+        sim.src[50, 170:230] = amp
+        sim.add_nl(2.8e-18, region)
+        _, _, E = sim.solve_fields()
+        eps_nl_max = float(np.max(3 * 2.8e-18 / 1e-12 * region * np.abs(E) ** 2))
+        _, _, _, conv = sim.born_solve(conv_threshold=1e-10, max_num_iter=n_steps)
+        out['%g' % amp] = {'eps_nl_max_linear_field': eps_nl_max, 'born_conv': [float(c) for c in conv.ravel()]}
+    with open(os.path.join(HERE, 'c5_stated_amplitudes.json'), 'w') as f:
+        json.dump(out, f, indent=1)
 
 
 if __name__ == '__main__':
-    main()
+    if sys.argv[1:] == ['c5']:
+        c5_stated()
+    else:
+        main()

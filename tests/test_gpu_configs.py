@@ -83,3 +83,27 @@ def test_c5_born_full_size():
     _, _, rE, rconv = ref.born_solve(conv_threshold=1e-10, max_num_iter=50)
     assert np.linalg.norm(E_top - rE) / np.linalg.norm(rE) <= 1e-9
     print(f"c5: Born steps per amplitude {steps}")
+
+
+def test_c5_born_at_stated_amplitude_does_not_converge(capsys):
+    """The stated amplitude 1e8: the reference's Born loop runs its 50 steps,
+    prints that it did not converge and returns the last iterate
+    (nonlinear_solvers.py:49-54); the relative changes follow the oracle's
+    2-cycle (tests/golden/c5_stated_amplitudes.json)."""
+    import json
+    import os
+    from conftest import GOLDEN
+    gold = json.load(open(os.path.join(GOLDEN, 'c5_stated_amplitudes.json')))['1e+08']['born_conv']
+    p = C.c5_kerr()
+    sim = Simulation(p.omega, p.eps, p.dl, p.npml, 'Ez')
+    sim.add_nl(p.extra['chi3'], p.extra['nl_region'])
+    sim.src = p.srcs[0] * 1e8
+    Hx, Hy, Ez, conv = sim.solve_fields_nl(solver_nl='born', conv_threshold=1e-10, max_num_iter=50)
+    c = conv.ravel()
+    out = capsys.readouterr().out
+    print(out)
+    print(f"c5 at amplitude 1e8: Born changes {c[:6]} ... {c[-2:]}")
+    assert "the simulation did not converge" in out
+    assert np.all(c > 1e-10) and np.all(np.isfinite(Ez))
+    np.testing.assert_allclose(c[:4], gold, rtol=1e-5)
+    np.testing.assert_allclose(c[-2:], gold[2:4], rtol=1e-5)

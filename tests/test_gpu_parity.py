@@ -5,6 +5,7 @@ import os
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 import torch
 
 from conftest import GOLDEN
@@ -12,7 +13,7 @@ from fdfdpy_amd import _lib as L
 from fdfdpy_amd import configs as C
 from fdfdpy_amd.device import DeviceOperator
 from fdfdpy_amd.simulation import Simulation
-from oracle import OracleSimulation, adjoint_gradient, solver_direct
+from oracle import EPSILON_0, OracleSimulation, adjoint_gradient, solver_direct
 
 pytestmark = pytest.mark.gpu
 OMEGA = C.omega_of(1.55e-6)
@@ -377,3 +378,41 @@ def test_flux_probe_on_device_matches_host_and_oracle(pol):
     # window clipped by the grid edge: host sum with numpy's slicing
     assert sim_t._flux_on_device(sim_t.fields, 'x', [59, 60], [20, 30]) is None
     assert sim_t.flux_probe('x', [59, 25], 10) == ref.flux_probe('x', [59, 25], 10)
+
+
+def test_include_nl_solves_with_the_kerr_diagonal_of_eps_nl():
+    """solve_fields(include_nl=True) solves A + Anl (simulation.py:121-124)
+    with Anl from sim.eps_nl, whatever the device operator held before: after
+    a nonlinear solve and a linear solve in between. compute_nl leaves
+    eps_nl, dnl_de, dnl_deps and the CSR Anl as the reference's does
+    (simulation.py:57-68)."""
+    N = (64, 64)
+    eps = np.ones(N)
+    eps[24:40, 24:40] = 12.25
+    region = np.zeros(N)
+    region[24:40, 24:40] = 1
+    sim = Simulation(OMEGA, eps, 0.02, [10, 10], 'Ez')
+    sim.src[12, 24:40] = 60.0
+    sim.add_nl(2.8e-18, region, eps_scale=True, eps_max=12.25)
+    ref = OracleSimulation(OMEGA, eps, 0.02, [10, 10], 'Ez')
+    ref.src[12, 24:40] = 60.0
+    ref.add_nl(2.8e-18, region, eps_scale=True, eps_max=12.25)
+    _, _, Enl, _ = sim.solve_fields_nl(solver_nl='born', conv_threshold=1e-10)
+    _, _, Elin = sim.solve_fields()               # the linear operator in between
+    Hx, Hy, Ez = sim.solve_fields(include_nl=True)
+    # the oracle's A + Anl with the same eps_nl
+    ref.eps_nl = np.asarray(sim.eps_nl)
+    Mn = N[0] * N[1]
+    ref.Anl = sp.diags(OMEGA ** 2 * EPSILON_0 * 1e-6 * ref.eps_nl.reshape(-1), 0, shape=(Mn, Mn), format='csr')
+    rHx, rHy, rEz = ref.solve_fields(include_nl=True)
+    assert relerr(Ez, rEz) <= 1e-10 and relerr(Hx, rHx) <= 1e-10
+    assert relerr(Ez, Elin) > 1e-6                 # the Kerr term matters here
+    # compute_nl's host state for a given field against the oracle's
+    e = np.asarray(Enl)
+    sim.compute_nl(e)
+    ref.compute_nl(e)
+    np.testing.assert_allclose(sim.eps_nl, ref.eps_nl, rtol=1e-9, atol=1e-14)
+    np.testing.assert_allclose(sim.dnl_de, ref.dnl_de, rtol=1e-12, atol=1e-14)
+    expect_deps = 3 * 2.8e-18 / 1e-12 * region * np.abs(e) ** 2 / (12.25 - 1)
+    np.testing.assert_allclose(sim.dnl_deps, expect_deps, rtol=1e-12, atol=1e-14)
+    assert abs(sim.Anl - ref.Anl).max() <= 1e-9 * abs(ref.Anl).max()

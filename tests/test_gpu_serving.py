@@ -106,3 +106,27 @@ def test_engine_batches_of_more_than_eight_jobs():
     for a, b in zip(ops_a, ops_b):
         assert torch.equal(a.factors["c64"][0], b.factors["c64"][0])
     eng.close()
+
+
+def test_large_requests_alone_equal_in_mix():
+    """The stream's 384 and 512 grids in a mix with smaller ones (two
+    factorization engines, c64 factors): each field is bit-identical to the
+    same request served alone, and its residual on the oracle's CSR operator
+    is within the tolerance."""
+    sizes = [384, 512, 128, 256, 384]
+    reqs = [Request(i, C.c3_request(200 + i, n=sizes[i]), 0.0) for i in range(len(sizes))]
+    mix = ServingEngine("cuda:0", precision="c64", slots=4, keep_fields=True).run(reqs)
+    assert all(not r.error for r in mix), [r.error for r in mix]
+    for target in (0, 1):
+        alone = ServingEngine("cuda:0", precision="c64", slots=1, keep_fields=True).run([reqs[target]])[0]
+        assert np.array_equal(alone.fields.cpu().numpy(), mix[target].fields.cpu().numpy())
+    for r, req in zip(mix, reqs):
+        p = req.problem
+        A, _ = construct_A(p.omega, [0, p.shape[0] * p.dl], [0, p.shape[1] * p.dl], p.eps, p.npml, p.pol, p.L0)
+        x = r.fields.cpu().numpy()
+        b = p.rhs()
+        worst = max(np.linalg.norm(A @ x[q].reshape(-1) - b[q].reshape(-1)) / np.linalg.norm(b[q])
+                    for q in range(b.shape[0]))
+        print(f"request {r.rid} ({p.shape[0]}^2, {b.shape[0]} RHS): {r.iterations} refinement steps, "
+              f"oracle-CSR residual {worst:.2e}")
+        assert worst <= 1.5e-10

@@ -37,3 +37,20 @@ def test_bad_inputs_raise_value_and_assertion_error(good, kw):
         make(good, **kw)
     with pytest.raises(InputError):
         make(good, **kw)
+
+
+def test_package_import_surface():
+    """from fdfdpy_amd import Simulation, as from fdfdpy import Simulation
+    (fdfdpy/__init__.py:3)."""
+    import fdfdpy_amd
+    from fdfdpy_amd import InputError as IE
+    from fdfdpy_amd import Simulation as S
+    assert S is Simulation and IE is InputError
+    assert fdfdpy_amd.name == "fdfdpy_amd"
+
+
+def test_grid_beyond_the_largest_block_raises_input_error():
+    """The direct solver's block is the shorter grid side, at most 2048 cells;
+    a larger grid is refused before any device work."""
+    with pytest.raises(InputError):
+        Simulation(1e15, np.ones((2056, 2050)), 0.02, [20, 20], 'Ez')
