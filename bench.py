@@ -13,16 +13,30 @@ each the equivalent of one reference solve_fields call) with p50/p99
 latency, for the complex64-factor + FP64-refinement path (the headline
 value) and for the FP64-factor path on the same trace.
 
+Two loads: the throughput leg offers the stream faster than it is served
+(Poisson arrivals at --rate), so solves/s is the saturated rate and its
+latencies include queueing; the latency leg replays the first
+--latency-requests requests at 0.7 x the FP64 path's measured request rate,
+below both paths' capacity, and its p50/p99 are the headline latencies.
+
 Also reported: the roofline of the dominant kernel (the Gauss-Jordan step
-of the complex64 factor, timed with HIP events on its stream) and a CPU
-baseline: the oracle's SuperLU restatement of the reference's 'scipy'
-branch on a bounded sample of the same stream.
+of the complex64 factor, 8 factorizations per launch as in the stream,
+timed with HIP events on its stream; HBM bytes per launch from two
+rocprofv3 --pmc passes over the same launches, scaled by a copy of known
+size) and a CPU baseline: the oracle's SuperLU restatement of the
+reference's 'scipy' branch in a pool of one worker per host core, replaying
+a bounded prefix of the same stream.
 """
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -30,11 +44,9 @@ sys.path.insert(0, ROOT)
 
 FP32_MFMA_PEAK_TFLOPS = 157.3
 FP64_MFMA_PEAK_TFLOPS = 78.6
-# HBM bytes per k_gj_step<float, false> launch at n = 512 with symmetric steps
-# (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, calibrated on a 256 MiB copy),
-# from the rocprofv3 --pmc runs of `bench.py --roofline-only` summarized in
-# profiles/k_gj_step_c64_512_roofline.md
-GJ_STEP_C64_512_TRAFFIC_BYTES = 3.461e6
+ROOF_N = 512          # grid of the roofline leg's factorizations
+ROOF_JOBS = 8         # factorizations per step launch (the stream runs 7-10)
+COPY_BYTES = 256 << 20
 
 
 def log(*a):
@@ -51,8 +63,16 @@ def parse():
     ap.add_argument("--slots", type=int, default=32, help="requests in flight")
     ap.add_argument("--engines", type=int, default=2, help="factorization engines (streams of step launches)")
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0,
-                    help="budget of the timed CPU sample (sizes beyond it are extrapolated)")
+    ap.add_argument("--cpu-seconds", type=float, default=25.0,
+                    help="wall-time budget of the CPU pool's sample of the stream")
+    ap.add_argument("--cpu-workers", type=int, default=0,
+                    help="CPU pool size (0: the host cores of this process, at most 16)")
+    ap.add_argument("--latency-requests", type=int, default=256,
+                    help="requests of the latency leg (arrivals below capacity)")
+    ap.add_argument("--latency-load", type=float, default=0.7,
+                    help="arrival rate of the latency leg as a fraction of the FP64 path's request rate")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 byte-counter passes")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--roofline-only", action="store_true", help="only the dominant-kernel measurement")
@@ -98,45 +118,173 @@ def _timed_wrap(op, precision):
     return e0.elapsed_time(e1)
 
 
-def roofline_leg(n=512, precision="c64"):
-    """Factor one n x n operator with the stepwise kernel the request loop uses.
-
-    The factorization is n*(n/32) back-to-back k_gj_step launches (+1 for
-    the first pivot), timed with HIP events on their stream; the x-wrap
-    correction that follows them in fdfd_factor is timed alone and taken out.
-    The single persistent launch (k_gj_persist, the single-request path) is
-    timed too.
-    """
+def _roof_ops(n, njobs):
+    """njobs operators of the stream's n x n grid (request seeds 7, 8, ...)."""
     from fdfdpy_amd import configs as C
     from fdfdpy_amd.device import DeviceOperator
-    p = C.c3_request(7, n=n)
-    op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, "cuda:0")
-    op.set_eps(p.eps)
-    ms_all = _timed_factor(op, precision, stepwise=True)
-    ms_wrap = _timed_wrap(op, precision)
+    ops = []
+    for j in range(njobs):
+        p = C.c3_request(7 + j, n=n)
+        op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, "cuda:0")
+        op.set_eps(p.eps)
+        ops.append(op)
+    return ops
+
+
+def _batched_factor(ops, precision, stream_timed=True):
+    """fdfd_factor_batch over ops (one grid) on the current stream, twice:
+    warm, then timed with HIP events on that stream. Returns ms."""
+    import ctypes
+    import torch
+    from fdfdpy_amd import _lib as L
+    from fdfdpy_amd.device import _stream
+    lib = L.lib()
+    g = ctypes.byref(ops[0].grid)
+    pr = L.PREC[precision]
+    nj = len(ops)
+    facs = [torch.empty(lib.fdfd_factor_bytes(g, pr), dtype=torch.uint8, device="cuda:0") for _ in ops]
+    works = [torch.empty(lib.fdfd_factor_workspace_bytes(g, pr), dtype=torch.uint8, device="cuda:0") for _ in ops]
+    arr = lambda ts: (ctypes.c_void_p * nj)(*[t.data_ptr() for t in ts])
+    a_ops, a_fac, a_work = arr([o.op for o in ops]), arr(facs), arr(works)
+    ms = None
+    for rep in range(2 if stream_timed else 1):
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(s)
+        L.check(lib.fdfd_factor_batch(g, a_ops, a_fac, nj, pr, a_work, _stream("cuda:0")))
+        e1.record(s)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+    for w in works:
+        L.check(lib.fdfd_factor_status(g, pr, ctypes.c_void_p(w.data_ptr()), _stream("cuda:0")))
+    for op, f in zip(ops, facs):
+        op.factors[precision] = (f, op.version, False)
+    return ms
+
+
+def roofline_leg(n=ROOF_N, precision="c64", njobs=ROOF_JOBS, traffic=None):
+    """The dominant kernel of the stream, k_gj_step, at the stream's batching.
+
+    njobs factorizations of n x n operators share each step launch
+    (fdfd_factor_batch: n*(n/32) + 1 launches), timed with HIP events on
+    their stream; the x-wrap corrections that follow them are timed alone
+    and taken out. Algorithmic flops per launch: njobs * 8 n^2 32 (the
+    8 m n^3 of one block factorization over its n*n/32 launches). The same
+    factorization with one job per launch (the launch-per-step path of a
+    single request) and the persistent single-launch factor are timed too.
+    """
+    ops = _roof_ops(n, njobs)
+    ms_all = _batched_factor(ops, precision)
+    ms_wrap = sum(_timed_wrap(op, precision) for op in ops)
     ms = ms_all - ms_wrap
-    ms_persist = _timed_factor(op, precision, stepwise=False) - ms_wrap
+    one = ops[0]
+    ms1 = _timed_factor(one, precision, stepwise=True) - ms_wrap / njobs
+    ms_persist = _timed_factor(one, precision, stepwise=False) - ms_wrap / njobs
     nt = (n + 31) // 32
     launches = n * nt + 1
+    flops_launch = 8.0 * n * n * 32
     flops = 8.0 * n ** 4                       # 8 m n^3 real flops, m = n blocks
-    achieved = flops / (ms * 1e-3) / 1e12
+    avg_us = ms * 1e3 / launches
+    achieved = njobs * flops_launch / (avg_us * 1e-6) / 1e12
     peak = FP32_MFMA_PEAK_TFLOPS if precision == "c64" else FP64_MFMA_PEAK_TFLOPS
     tname = "float" if precision == "c64" else "double"
-    return {
+    out = {
         "kernel": f"k_gj_step<{tname}, false>",
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-        "frac": round(achieved / peak, 5),
-        "traffic": GJ_STEP_C64_512_TRAFFIC_BYTES if (n, precision) == (512, "c64") else None,
-        "workload": f"complex factor of a {n}x{n} Ez operator: {launches} launches, "
-                    f"8*n^4 = {flops:.3e} flops per factor, 8*n^2*32 = {8.0 * n * n * 32:.3e} per launch",
-        "avg_launch_us": round(ms * 1e3 / launches, 3), "factor_ms": round(ms, 3),
-        "wrap_ms": round(ms_wrap, 3),
+        "frac": round(achieved / peak, 5), "traffic": None,
+        "workload": f"{njobs} complex factors of {n}x{n} Ez operators sharing each step launch: {launches} "
+                    f"launches of {njobs} x 8*n^2*32 = {njobs * flops_launch:.3e} flops",
+        "avg_launch_us": round(avg_us, 3), "batched_factor_ms": round(ms, 3), "wrap_ms": round(ms_wrap, 3),
+        "hbm_gbs_algorithmic": None,
+        "one_job_per_launch": {"avg_launch_us": round(ms1 * 1e3 / launches, 3), "factor_ms": round(ms1, 3),
+                               "achieved_tflops": round(flops / (ms1 * 1e-3) / 1e12, 3)},
         "persistent_factor_ms": round(ms_persist, 3),
         "persistent_achieved_tflops": round(flops / (ms_persist * 1e-3) / 1e12, 3),
     }
+    # algorithmic bytes per launch: the upper tiles of every job's working
+    # matrix read and written once (symmetric steps), plus the tiles below the
+    # diagonal written at each block's last step
+    es = 8 if precision == "c64" else 16
+    upper = nt * (nt + 1) // 2 * 32 * 32 * es * 2
+    lower = nt * (nt - 1) // 2 * 32 * 32 * es / nt
+    alg = njobs * (upper + lower)
+    out["algorithmic_bytes_per_launch"] = round(alg)
+    out["hbm_gbs_algorithmic"] = round(alg / (avg_us * 1e-6) / 1e9, 1)
+    if traffic:
+        out["traffic"] = traffic.get("bytes_per_launch")
+        out["traffic_detail"] = traffic
+    return out
 
 
-def copy_calibration(nbytes=256 << 20, reps=3):
+def traffic_child(n=ROOF_N, precision="c64", njobs=ROOF_JOBS):
+    """Run under rocprofv3 --pmc: a plain copy of known size (three times),
+    then the roofline leg's batched factorization once."""
+    import torch
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    _batched_factor(_roof_ops(n, njobs), precision, stream_timed=False)
+    print("traffic child done", flush=True)
+
+
+def _pmc_means(counter, timeout_s=240):
+    """Mean per-dispatch value of one counter (KiB) for the step kernel and
+    the copy kernel, from one rocprofv3 --pmc pass over traffic_child()."""
+    rp = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(rp):
+        raise RuntimeError("rocprofv3 not found")
+    d = tempfile.mkdtemp(prefix="fdfd_pmc_")
+    try:
+        cmd = [rp, "--pmc", counter, "-d", d, "-o", "pmc", "--output-format", "csv", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-child"]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=timeout_s,
+                           env=dict(os.environ, TMPDIR=d))
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            tail = r.stdout.decode(errors="replace")[-400:]
+            raise RuntimeError(f"rocprofv3 wrote no counter file (rc {r.returncode}): {tail}")
+        per = {}
+        for f in files:
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    key = (row["Kernel_Name"], row.get("Dispatch_Id", ""))
+                    per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+        step = [v for (k, _), v in per.items() if k.startswith("void k_gj_step<float, false>")
+                or k.startswith("k_gj_step<float, false>")]
+        copy = [v for (k, _), v in per.items() if "copyBuffer" in k]
+        if not step or not copy:
+            raise RuntimeError(f"counter rows missing (step {len(step)}, copy {len(copy)})")
+        return sum(step) / len(step), len(step), max(copy)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def traffic_leg():
+    """HBM bytes per batched step launch: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) in two rocprofv3 --pmc passes, each scaled by what the counter
+    reads for the 256 MiB copy. Runs before this process touches the GPU
+    (the passes are child processes)."""
+    try:
+        f_kib, nd, f_copy = _pmc_means("FETCH_SIZE")
+        w_kib, _, w_copy = _pmc_means("WRITE_SIZE")
+    except Exception as e:   # reported, the rest of the bench goes on
+        return {"bytes_per_launch": None, "error": f"{type(e).__name__}: {e}"[:300]}
+    copy_kib = COPY_BYTES / 1024
+    cf, cw = copy_kib / f_copy, copy_kib / w_copy
+    return {"bytes_per_launch": round((f_kib * cf + w_kib * cw) * 1024), "dispatches": nd,
+            "fetch_kib": round(f_kib, 1), "write_kib": round(w_kib, 1),
+            "fetch_scale": round(cf, 3), "write_scale": round(cw, 3),
+            "calibration": f"{COPY_BYTES >> 20} MiB device copy: FETCH_SIZE {f_copy:.0f} KiB, "
+                           f"WRITE_SIZE {w_copy:.0f} KiB"}
+
+
+def copy_calibration(nbytes=COPY_BYTES, reps=3):
     """A plain device copy of known size (scales the PMC byte counters)."""
     import torch
     a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
@@ -147,50 +295,76 @@ def copy_calibration(nbytes=256 << 20, reps=3):
     return {"copy_bytes_each_way": nbytes, "copies": reps}
 
 
-def cpu_baseline(reqs, seconds):
+def _cpu_solve(task):
+    """One reference-style solve in a pool worker: CSR assembly + SuperLU
+    (linalg.py:41-107, :140-141) for one right-hand side of a stream request."""
+    seed, q = task
+    from threadpoolctl import threadpool_limits
+    from fdfdpy_amd import configs as C
+    from oracle import construct_A, solver_direct
+    with threadpool_limits(limits=1):
+        p = C.c3_request(seed)
+        t = time.perf_counter()
+        A, _ = construct_A(p.omega, [0, p.shape[0] * p.dl], [0, p.shape[1] * p.dl], p.eps, p.npml, p.pol, p.L0)
+        solver_direct(A, p.srcs[q] * 1j * p.omega)
+        return seed, q, time.perf_counter() - t, time.perf_counter()
+
+
+def cpu_baseline(reqs, seconds, workers=0):
     """The oracle's restatement of the reference 'scipy' path (CSR assembly +
     SuperLU spsolve, refactored per right-hand side as linalg.py:122-149
-    does), one host core, on a bounded sample of the stream: one solve of the
-    first request of each grid size, smallest first, while the sample stays
-    within `seconds`; sizes left out are extrapolated with the growth
-    exponent of the two largest timed sizes. The stream's CPU rate is its
-    solves divided by the sum of per-solve times over its requests."""
-    import math
-    from threadpoolctl import threadpool_limits
-    from oracle import construct_A, solver_direct
-    first = {}
-    for r in reqs:
-        first.setdefault(r.problem.shape[0], r.problem)
-    sizes = sorted(first)
-    per_solve, spent = {}, 0.0
-    with threadpool_limits(limits=1):
-        for n in sizes:
-            if per_solve and spent >= seconds:
+    does) in a pool of one single-threaded worker per host core, replaying a
+    prefix of the stream in arrival order (arrival times as in the stream).
+    The prefix is the longest whose estimated work fits `seconds` of wall
+    time on the pool, from one timed 192 x 192 solve and a cubic growth in n.
+    Reports solves/s, p50/p99 latency (arrival -> last RHS done, queueing
+    included) and p50/p99 service time (first RHS started -> last done)."""
+    import multiprocessing as mp
+    import numpy as np
+    W = workers or min(len(os.sched_getaffinity(0)), 16)
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(W) as pool:
+        # calibration (also warms every worker's imports)
+        probe = [r for r in reqs if r.problem.shape[0] == 192][:1]
+        pool.map(_cpu_solve, [(0, 0)] * W)
+        t192 = pool.apply(_cpu_solve, ((probe[0].rid if probe else 0, 0),))[2] if probe else 0.4
+        budget = W * seconds
+        sample, work = [], 0.0
+        for r in reqs:
+            n = r.problem.shape[0]
+            w = r.problem.srcs.shape[0] * t192 * (n / 192.0) ** 3
+            if sample and work + w > budget:
                 break
-            p = first[n]
-            t = time.perf_counter()
-            A, _ = construct_A(p.omega, [0, n * p.dl], [0, p.shape[1] * p.dl], p.eps, p.npml, p.pol, p.L0)
-            solver_direct(A, p.srcs[0] * 1j * p.omega)
-            per_solve[n] = time.perf_counter() - t
-            spent += per_solve[n]
-            log(f"  cpu: {n}x{n} solve {per_solve[n]:.2f} s")
-    timed = sorted(per_solve)
-    extrap = [n for n in sizes if n not in per_solve]
-    if extrap:
-        if len(timed) >= 2:
-            a, b = timed[-2], timed[-1]
-            expo = math.log(per_solve[b] / per_solve[a]) / math.log(b / a)
-        else:
-            expo = 3.0
-        for n in extrap:
-            per_solve[n] = per_solve[timed[-1]] * (n / timed[-1]) ** expo
-    total = sum(r.problem.srcs.shape[0] * per_solve[r.problem.shape[0]] for r in reqs)
-    nsolves = sum(r.problem.srcs.shape[0] for r in reqs)
-    note = (f"; {extrap} extrapolated with exponent {expo:.2f} from {timed[-2:]}" if extrap else "")
-    return {"value": round(nsolves / total, 5), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"one timed solve per grid size {timed} ({spent:.1f} s on 1 host core){note}; "
-                      f"rate = {nsolves} stream solves / sum of per-solve times",
-            "per_solve_s": {str(n): round(per_solve[n], 3) for n in sizes}}
+            sample.append(r)
+            work += w
+        t0 = time.perf_counter()
+        asyncs = []
+        for r in sample:
+            wait = r.arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            for q in range(r.problem.srcs.shape[0]):
+                asyncs.append((r, pool.apply_async(_cpu_solve, ((r.rid, q),))))
+        done = {}
+        for r, a in asyncs:
+            seed, q, dt, tend = a.get()
+            done.setdefault(r.rid, []).append((dt, tend))
+        wall = time.perf_counter() - t0
+    lat, svc = [], []
+    for r in sample:
+        ends = [te for _, te in done[r.rid]]
+        lat.append(max(ends) - t0 - r.arrival)
+        svc.append(max(ends) - min(te - dt for dt, te in done[r.rid]))
+    nsolves = sum(r.problem.srcs.shape[0] for r in sample)
+    sizes = sorted({r.problem.shape[0] for r in sample})
+    return {"value": round(nsolves / wall, 4), "unit": "solves/s", "cores": W, "kind": "port",
+            "sample": f"first {len(sample)} requests of the stream ({nsolves} solves, grids {sizes}) replayed "
+                      f"at their arrival times on a pool of {W} single-threaded workers, one SuperLU refactor "
+                      f"per RHS; {wall:.1f} s wall",
+            "p50_latency_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+            "p99_latency_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
+            "p50_service_ms": round(float(np.percentile(svc, 50)) * 1e3, 1),
+            "p99_service_ms": round(float(np.percentile(svc, 99)) * 1e3, 1)}
 
 
 def _sync_time(fn):
@@ -298,6 +472,17 @@ def run_stream(reqs, precision, slots, tol, engines=2):
     return summ, res
 
 
+def _path_summary(summ):
+    return {"solves_per_s": round(summ["solves_per_s"], 4),
+            "requests_per_s": round(summ["requests_per_s"], 4),
+            "p50_latency_ms": round(summ["p50_latency_s"] * 1e3, 3),
+            "p99_latency_ms": round(summ["p99_latency_s"] * 1e3, 3),
+            "p50_service_ms": round(summ["p50_service_s"] * 1e3, 3),
+            "p99_service_ms": round(summ["p99_service_s"] * 1e3, 3),
+            "max_rel_residual": summ["max_rel_residual"], "failed": summ["failed"],
+            "fallbacks": summ["fallbacks"], "mean_sweeps": summ["mean_sweeps"]}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -305,11 +490,34 @@ def main():
         print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1 "
                                    "(multi-GPU sharding is a design, not built)"}))
         sys.exit(2)
+    if args.traffic_child:
+        traffic_child()
+        return
     import numpy as np
-    import torch
     from fdfdpy_amd import configs as C
     from fdfdpy_amd.serving import Request
 
+    n_req = args.steps * args.batch
+    problems = [C.c3_request(s) for s in range(n_req)]
+    rng = np.random.default_rng(3)
+    gaps = rng.exponential(1.0 / args.rate, size=n_req)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs = [Request(i, problems[i], float(arrivals[i])) for i in range(n_req)]
+
+    # host-only legs first: their child processes start before this process
+    # initializes the GPU
+    traffic = None
+    if not (args.no_traffic or args.roofline_only or args.copy_calibration or args.configs):
+        log("traffic: two rocprofv3 --pmc passes over the roofline leg's launches")
+        traffic = traffic_leg()
+        log(f"traffic: {json.dumps(traffic)}")
+    cpu = None
+    if not (args.no_cpu_baseline or args.roofline_only or args.copy_calibration or args.configs):
+        log("cpu: pool replay of a prefix of the stream")
+        cpu = cpu_baseline(reqs, args.cpu_seconds, args.cpu_workers)
+        log(f"cpu: {json.dumps(cpu)}")
+
+    import torch
     if args.roofline_only:
         print(json.dumps(roofline_leg()))
         return
@@ -320,14 +528,7 @@ def main():
         print(json.dumps(config_timings()))
         return
 
-    n_req = args.steps * args.batch
     log(f"stream: {n_req} requests, Poisson rate {args.rate}/s, {args.slots} slots")
-    problems = [C.c3_request(s) for s in range(n_req)]
-    rng = np.random.default_rng(3)
-    gaps = rng.exponential(1.0 / args.rate, size=n_req)
-    arrivals = np.cumsum(gaps) - gaps[0]
-    reqs = [Request(i, problems[i], float(arrivals[i])) for i in range(n_req)]
-
     # warmup: separate requests of the same stream (8 per warmup step)
     for w in range(args.warmup):
         wreqs = [Request(i, C.c3_request(100000 + 8 * w + i), 0.0) for i in range(8)]
@@ -335,17 +536,30 @@ def main():
             s, _ = run_stream(wreqs, prec, args.slots, args.tol, args.engines)
             log(f"warmup {w} {prec}: {s['solves']} solves in {s['wall_s']:.2f}s")
 
-    mixed, res_m = run_stream(reqs, "c64", args.slots, args.tol, args.engines)
+    # throughput leg: the whole stream, offered faster than it is served
+    mixed, _ = run_stream(reqs, "c64", args.slots, args.tol, args.engines)
     log(f"mixed: {json.dumps(mixed)}")
     fp64 = None
     if not args.no_fp64:
-        fp64, res_f = run_stream(reqs, "fp64", args.slots, args.tol, args.engines)
+        fp64, _ = run_stream(reqs, "fp64", args.slots, args.tol, args.engines)
         log(f"fp64: {json.dumps(fp64)}")
-    roof = roofline_leg()
+    # latency leg: a prefix of the stream at 0.7 x the FP64 path's request rate
+    cap = (fp64 or mixed)["requests_per_s"]
+    lat_rate = args.latency_load * cap
+    k = min(args.latency_requests, n_req)
+    lrng = np.random.default_rng(33)
+    lgaps = lrng.exponential(1.0 / lat_rate, size=k)
+    larr = np.cumsum(lgaps) - lgaps[0]
+    lreqs = [Request(i, problems[i], float(larr[i])) for i in range(k)]
+    log(f"latency leg: {k} requests at {lat_rate:.2f}/s")
+    lat_m, _ = run_stream(lreqs, "c64", args.slots, args.tol, args.engines)
+    log(f"latency mixed: {json.dumps(lat_m)}")
+    lat_f = None
+    if not args.no_fp64:
+        lat_f, _ = run_stream(lreqs, "fp64", args.slots, args.tol, args.engines)
+        log(f"latency fp64: {json.dumps(lat_f)}")
+    roof = roofline_leg(traffic=traffic)
     log(f"roofline: {json.dumps(roof)}")
-    cpu = None if args.no_cpu_baseline else cpu_baseline(reqs, args.cpu_seconds)
-    if cpu:
-        log(f"cpu: {json.dumps(cpu)}")
 
     ms_per_step = mixed["wall_s"] * 1e3 / args.steps
     out = {
@@ -360,27 +574,26 @@ def main():
         "config": {"workload": f"serving stream: {n_req} Ez requests, grids 128-768, Poisson arrivals "
                                f"{args.rate}/s, 1-4 RHS each, residual {args.tol:g}",
                    "requests": n_req, "batch": args.batch, "slots": args.slots, "engines": args.engines,
+                   "latency_leg": f"first {k} requests, Poisson arrivals at {lat_rate:.2f}/s = "
+                                  f"{args.latency_load} x the FP64 path's {cap:.2f} requests/s",
                    "parallelism": "single GPU"},
-        "p50_latency_ms": round(mixed["p50_latency_s"] * 1e3, 3),
-        "p99_latency_ms": round(mixed["p99_latency_s"] * 1e3, 3),
-        "p50_service_ms": round(mixed["p50_service_s"] * 1e3, 3),
-        "p99_service_ms": round(mixed["p99_service_s"] * 1e3, 3),
-        "max_rel_residual": mixed["max_rel_residual"], "fallbacks": mixed["fallbacks"],
-        "failed": mixed["failed"], "mean_sweeps": mixed["mean_sweeps"],
-        "roofline": {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
-        "roofline_detail": {k: roof[k] for k in ("kernel", "workload", "avg_launch_us", "factor_ms",
-                                                 "persistent_factor_ms", "persistent_achieved_tflops")},
+        "p50_latency_ms": round(lat_m["p50_latency_s"] * 1e3, 3),
+        "p99_latency_ms": round(lat_m["p99_latency_s"] * 1e3, 3),
+        "p50_service_ms": round(lat_m["p50_service_s"] * 1e3, 3),
+        "p99_service_ms": round(lat_m["p99_service_s"] * 1e3, 3),
+        "max_rel_residual": max(mixed["max_rel_residual"], lat_m["max_rel_residual"]),
+        "fallbacks": mixed["fallbacks"], "failed": mixed["failed"] + lat_m["failed"],
+        "mean_sweeps": mixed["mean_sweeps"],
+        "saturated": _path_summary(mixed) | {"engine_launches": mixed.get("engine_launches"),
+                                             "jobs_per_launch": mixed.get("jobs_per_launch")},
+        "roofline": {k_: roof[k_] for k_ in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "roofline_detail": {k_: v for k_, v in roof.items()
+                            if k_ not in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "cpu_baseline": cpu,
     }
     if fp64:
-        out["fp64_path"] = {
-            "solves_per_s": round(fp64["solves_per_s"], 4),
-            "p50_latency_ms": round(fp64["p50_latency_s"] * 1e3, 3),
-            "p99_latency_ms": round(fp64["p99_latency_s"] * 1e3, 3),
-            "p50_service_ms": round(fp64["p50_service_s"] * 1e3, 3),
-            "p99_service_ms": round(fp64["p99_service_s"] * 1e3, 3),
-            "max_rel_residual": fp64["max_rel_residual"], "failed": fp64["failed"],
-            "mean_sweeps": fp64["mean_sweeps"]}
+        out["fp64_path"] = {"saturated": _path_summary(fp64),
+                            "latency_leg": _path_summary(lat_f) if lat_f else None}
         out["mixed_over_fp64"] = round(mixed["solves_per_s"] / fp64["solves_per_s"], 4)
     print(json.dumps(out), flush=True)
 

@@ -65,6 +65,8 @@ SIGNATURES = {
     "fdfd_apply": (c_int, [_G, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "fdfd_residual": (c_int, [_G, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "fdfd_factor": (c_int, [_G, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "fdfd_factor_batch": (c_int, [_G, POINTER(c_void_p), POINTER(c_void_p), c_int, c_int, POINTER(c_void_p),
+                                  c_void_p]),
     "fdfd_factor_wrap": (c_int, [_G, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "fdfd_factor_status": (c_int, [_G, c_int, c_void_p, c_void_p]),
     "fdfd_sweep": (c_int, [_G, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),

@@ -2130,6 +2130,35 @@ int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, 
   return FDFD_OK;
 }
 
+// The block factorizations of njobs operators of one grid in shared step
+// launches (the request loop's batched launches, on the caller's stream).
+template <typename T, bool PIV>
+int factor_batch_impl(const fdfd_grid *g, const double *const *ops, void *const *factors, void *const *works,
+                      int njobs, hipStream_t s) {
+  const int n = g->ny, np = pad_n(n), nt = np / TB, m = g->nx;
+  FBatch<T> b;
+  memset(&b, 0, sizeof b);
+  b.njobs = njobs;
+  for (int j = 0; j < njobs; ++j) {
+    FactorWork fw = factor_work_layout(g, sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64, works[j]);
+    HIP_TRY(hipMemsetAsync(fw.status, 0, sizeof(int), s));
+    FJob<T> &J = b.job[j];
+    J.op = ops[j]; J.G = (T *)factors[j]; J.W = (T *)fw.W; J.Pinv = (T *)fw.Pinv; J.status = fw.status;
+    J.dsym = g_step_sym.load() ? (T *)fw.xcol : nullptr;
+    J.m = m; J.n = n; J.np = np; J.nt = nt;
+    J.k = -1; J.p = 0;
+  }
+  int rc = launch_step<T, PIV>(b, s);
+  if (rc) return rc;
+  for (int k = 0; k < m; ++k) {
+    for (int p = 0; p < nt; ++p) {
+      for (int j = 0; j < njobs; ++j) { b.job[j].k = k; b.job[j].p = p; }
+      if ((rc = launch_step<T, PIV>(b, s))) return rc;
+    }
+  }
+  return FDFD_OK;
+}
+
 template <typename T>
 int sweep_blockwise(const fdfd_grid *g, const double *op, const void *factor, const double *r,
                     double *x, int nrhs, void *y, hipStream_t s) {
@@ -2630,6 +2659,28 @@ int fdfd_factor(const fdfd_grid *g, const double *op, void *factor, int precisio
               : wrap_setup<float>(g, op, factor, work, piv, stream);
 }
 
+int fdfd_factor_batch(const fdfd_grid *g, const double *const *ops, void *const *factors, int njobs,
+                      int precision, void *const *works, hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (int rc = check_prec(precision)) return rc;
+  if (!ops || !factors || !works || njobs < 1 || njobs > MAXJ)
+    return fail(FDFD_ERR_ARG, "bad arguments to fdfd_factor_batch (1 <= njobs <= %d)", MAXJ);
+  for (int j = 0; j < njobs; ++j)
+    if (!ops[j] || !factors[j] || !works[j]) return fail(FDFD_ERR_ARG, "fdfd_factor_batch: job %d has a NULL pointer", j);
+  const bool piv = precision & FDFD_FACTOR_PIVOT;
+  const bool fp64 = base_prec(precision) == FDFD_PREC_FP64;
+  int rc = fp64 ? (piv ? factor_batch_impl<double, true>(g, ops, factors, works, njobs, stream)
+                       : factor_batch_impl<double, false>(g, ops, factors, works, njobs, stream))
+                : (piv ? factor_batch_impl<float, true>(g, ops, factors, works, njobs, stream)
+                       : factor_batch_impl<float, false>(g, ops, factors, works, njobs, stream));
+  if (rc) return rc;
+  for (int j = 0; j < njobs; ++j) {
+    rc = fp64 ? wrap_setup<double>(g, ops[j], factors[j], works[j], piv, stream)
+              : wrap_setup<float>(g, ops[j], factors[j], works[j], piv, stream);
+    if (rc) return rc;
+  }
+  return FDFD_OK;
+}
 int fdfd_factor_wrap(const fdfd_grid *g, const double *op, void *factor, int precision, void *work,
                      hipStream_t stream) {
   if (int rc = check_grid(g)) return rc;

@@ -141,6 +141,16 @@ int fdfd_residual(const fdfd_grid *g, const double *op, const double *b,
 int fdfd_factor(const fdfd_grid *g, const double *op, void *factor, int precision,
                 void *work, hipStream_t stream);
 
+/* fdfd_factor for njobs (1..16) operators of the same grid: their block
+ * factorizations advance together, one shared Gauss-Jordan step launch per
+ * step (the launch-per-step path, as the batching engine runs it), then each
+ * wrap correction; all on stream. ops, factors and works are host arrays of
+ * njobs device pointers, sized as for fdfd_factor. Numeric status per job by
+ * fdfd_factor_status. Replaces njobs separate pardisoSolver(A).factor()
+ * calls (linalg.py:135-136). */
+int fdfd_factor_batch(const fdfd_grid *g, const double *const *ops, void *const *factors, int njobs,
+                      int precision, void *const *works, hipStream_t stream);
+
 /* The x-wrap correction step of fdfd_factor alone, for a block
  * factorization completed by the batching engine (same arguments; enqueued
  * on stream). No reference counterpart: part of the factor() replacement. */
