@@ -16,7 +16,7 @@ value) and for the FP64-factor path on the same trace.
 Two loads: the throughput leg offers the stream faster than it is served
 (Poisson arrivals at --rate), so solves/s is the saturated rate and its
 latencies include queueing; the latency leg replays the first
---latency-requests requests at 0.7 x the FP64 path's measured request rate,
+--latency-requests requests at 0.5 x the FP64 path's measured request rate,
 below both paths' capacity, and its p50/p99 are the headline latencies.
 
 Also reported: the roofline of the dominant kernel (the Gauss-Jordan step
@@ -69,10 +69,15 @@ def parse():
                     help="CPU pool size (0: the host cores of this process, at most 16)")
     ap.add_argument("--latency-requests", type=int, default=256,
                     help="requests of the latency leg (arrivals below capacity)")
-    ap.add_argument("--latency-load", type=float, default=0.7,
+    ap.add_argument("--latency-load", type=float, default=0.5,
                     help="arrival rate of the latency leg as a fraction of the FP64 path's request rate")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 byte-counter passes")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--stream-only", action="store_true",
+                    help="only the throughput leg of the stream (for kernel traces of it)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the kernels of the roofline leg (both precisions) and a 512 x 512 solve with "
+                         "4 right-hand sides in both precisions (for counter runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--roofline-only", action="store_true", help="only the dominant-kernel measurement")
@@ -231,6 +236,27 @@ def traffic_child(n=ROOF_N, precision="c64", njobs=ROOF_JOBS):
     print("traffic child done", flush=True)
 
 
+def kernels_only(n=ROOF_N, njobs=ROOF_JOBS, nrhs=4):
+    """The copy calibration, the roofline leg's batched factorization in both
+    precisions, and one n x n solve with nrhs right-hand sides to 1e-10 in
+    both precisions (persistent sweeps, wrap correction, FP64 stencil
+    residuals): the kernels whose counters the profiles report."""
+    import torch
+    copy_calibration()
+    ops = _roof_ops(n, njobs)
+    rng = torch.Generator(device="cpu").manual_seed(1)
+    out = {}
+    for prec in ("c64", "fp64"):
+        out[prec + "_batched_factor_ms"] = round(_batched_factor(ops, prec, stream_timed=False), 3)
+        b = torch.randn((nrhs,) + ops[0].shape, dtype=torch.complex128, generator=rng).to("cuda:0")
+        x, rep = ops[0].solve(b, precision=prec, tol=1e-10)
+        out[prec + "_solve"] = {"steps": rep.iterations, "residual": rep.max_rel_residual}
+        for op in ops:
+            op.drop_factor()
+    torch.cuda.synchronize()
+    return out
+
+
 def _pmc_means(counter, timeout_s=240):
     """Mean per-dispatch value of one counter (KiB) for the step kernel and
     the copy kernel, from one rocprofv3 --pmc pass over traffic_child()."""
@@ -255,11 +281,12 @@ def _pmc_means(counter, timeout_s=240):
                         continue
                     key = (row["Kernel_Name"], row.get("Dispatch_Id", ""))
                     per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
-        step = [v for (k, _), v in per.items() if k.startswith("void k_gj_step<float, false>")
-                or k.startswith("k_gj_step<float, false>")]
+        # demangled ("k_gj_step<float, false>") or mangled ("k_gj_stepIfLb0E") names
+        step = [v for (k, _), v in per.items() if "k_gj_step<float, false>" in k or "k_gj_stepIfLb0E" in k]
         copy = [v for (k, _), v in per.items() if "copyBuffer" in k]
         if not step or not copy:
-            raise RuntimeError(f"counter rows missing (step {len(step)}, copy {len(copy)})")
+            names = sorted({k for k, _ in per})[:6]
+            raise RuntimeError(f"counter rows missing (step {len(step)}, copy {len(copy)}; kernels {names})")
         return sum(step) / len(step), len(step), max(copy)
     finally:
         shutil.rmtree(d, ignore_errors=True)
@@ -507,12 +534,13 @@ def main():
     # host-only legs first: their child processes start before this process
     # initializes the GPU
     traffic = None
-    if not (args.no_traffic or args.roofline_only or args.copy_calibration or args.configs):
+    only = args.roofline_only or args.copy_calibration or args.configs or args.kernels_only or args.stream_only
+    if not (args.no_traffic or only):
         log("traffic: two rocprofv3 --pmc passes over the roofline leg's launches")
         traffic = traffic_leg()
         log(f"traffic: {json.dumps(traffic)}")
     cpu = None
-    if not (args.no_cpu_baseline or args.roofline_only or args.copy_calibration or args.configs):
+    if not (args.no_cpu_baseline or only):
         log("cpu: pool replay of a prefix of the stream")
         cpu = cpu_baseline(reqs, args.cpu_seconds, args.cpu_workers)
         log(f"cpu: {json.dumps(cpu)}")
@@ -526,6 +554,9 @@ def main():
         return
     if args.configs:
         print(json.dumps(config_timings()))
+        return
+    if args.kernels_only:
+        print(json.dumps(kernels_only()))
         return
 
     log(f"stream: {n_req} requests, Poisson rate {args.rate}/s, {args.slots} slots")
@@ -543,7 +574,10 @@ def main():
     if not args.no_fp64:
         fp64, _ = run_stream(reqs, "fp64", args.slots, args.tol, args.engines)
         log(f"fp64: {json.dumps(fp64)}")
-    # latency leg: a prefix of the stream at 0.7 x the FP64 path's request rate
+    if args.stream_only:
+        print(json.dumps({"stream_c64": mixed, "stream_fp64": fp64}), flush=True)
+        return
+    # latency leg: a prefix of the stream at 0.5 x the FP64 path's request rate
     cap = (fp64 or mixed)["requests_per_s"]
     lat_rate = args.latency_load * cap
     k = min(args.latency_requests, n_req)

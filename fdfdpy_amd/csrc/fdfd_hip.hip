@@ -480,54 +480,103 @@ __device__ inline void fast_crecip(T zr, T zi, T &outr, T &outi) {
   outi = -ai * rd;
 }
 
+// Exchange buffers of the tile inverse: rows padded to a 16-byte multiple so
+// that a thread's four consecutive elements move as one 16-byte LDS access
+// (complex64: 36 floats; FP64: 34 doubles). Two of them overlay the whole
+// Lds block while the tile itself is held in registers.
+template <typename T> constexpr int inv_stride() { return sizeof(T) == 8 ? 34 : 36; }
+template <typename T> struct InvX {
+  T r[TB][inv_stride<T>()];
+  T i[TB][inv_stride<T>()];
+};
+__device__ inline void lds_st4(float *p, const float (&v)[4]) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  *reinterpret_cast<f4 *>(p) = f4{v[0], v[1], v[2], v[3]};
+}
+__device__ inline void lds_st4(double *p, const double (&v)[4]) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  reinterpret_cast<d2 *>(p)[0] = d2{v[0], v[1]};
+  reinterpret_cast<d2 *>(p)[1] = d2{v[2], v[3]};
+}
+__device__ inline void lds_ld4(const float *p, float (&v)[4]) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 x = *reinterpret_cast<const f4 *>(p);
+  v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+}
+__device__ inline void lds_ld4(const double *p, double (&v)[4]) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 x = reinterpret_cast<const d2 *>(p)[0], y = reinterpret_cast<const d2 *>(p)[1];
+  v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
+}
+template <typename T>
+__device__ inline void lds_ld2(const T *p, T &a, T &b) {
+  typedef T t2 __attribute__((ext_vector_type(2)));
+  const t2 x = *reinterpret_cast<const t2 *>(p);
+  a = x.x;
+  b = x.y;
+}
+// 2^-e with 2^(e-1) <= |x| < 2^e: an exact scale that brings x to [0.5, 1)
+__device__ inline float exp_scale(float x) { return ldexpf(1.0f, -__builtin_amdgcn_frexp_expf(x)); }
+__device__ inline double exp_scale(double x) { return ldexp(1.0, -__builtin_amdgcn_frexp_exp(x)); }
+
 // Inverse of the TB x TB tile in (Ar, Ai), in place, by Gauss-Jordan with
 // 2 x 2 diagonal block pivots (the default; the tile is a diagonal block of
 // a Schur complement of the PML-damped operator): TB / 2 steps, each inverting the pivot block
 // in closed form and applying a rank-2 update. Thread t keeps row t/8,
 // columns 4(t%8)..+3 in registers and publishes them each step to one of two
-// LDS buffers, so one barrier per step orders the exchange; every LDS read of
-// a step is issued right after that barrier, and the pivot rows are handled
-// by the same update formula as the other rows (no divergent branch).
+// LDS exchange buffers, so one barrier per step orders the exchange; a step
+// reads the pivot block, its row's pivot columns and the pivot rows at its
+// columns with ten LDS accesses, and the pivot rows are handled by the same
+// update formula as the other rows (no divergent branch). Scales are exact
+// powers of two.
 template <typename T>
 __device__ void tile_inverse_diag(Lds<T> &L, int *status) {
   static_assert(TB * TB == 4 * NT, "4 elements per thread");
+  static_assert(2 * sizeof(InvX<T>) <= sizeof(Lds<T>), "exchange buffers overlay the Lds block");
   const int tid = threadIdx.x;
   const int i = tid >> 3, c0 = (tid & 7) * 4;
   T ar[4], ai[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) { ar[q] = L.Ar[i][c0 + q]; ai[q] = L.Ai[i][c0 + q]; }
+  __syncthreads();   // every row is in registers before the exchange buffers cover Ar, Ai
+  InvX<T> *X = reinterpret_cast<InvX<T> *>(&L);
   bool bad = false;
 #pragma unroll 2
   for (int j = 0; j < TB; j += 2) {
-    T (*Br)[TBP] = (j & 2) ? L.Br : L.Pr;
-    T (*Bi)[TBP] = (j & 2) ? L.Bi : L.Pi;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) { Br[i][c0 + q] = ar[q]; Bi[i][c0 + q] = ai[q]; }
+    InvX<T> &B = X[(j >> 1) & 1];
+    lds_st4(&B.r[i][c0], ar);
+    lds_st4(&B.i[i][c0], ai);
     __syncthreads();
     // pivot block, this row's pivot columns, the pivot rows at these columns
-    const T p00r = Br[j][j], p00i = Bi[j][j], p01r = Br[j][j + 1], p01i = Bi[j][j + 1];
-    const T p10r = Br[j + 1][j], p10i = Bi[j + 1][j], p11r = Br[j + 1][j + 1], p11i = Bi[j + 1][j + 1];
-    const T f0r = Br[i][j], f0i = Bi[i][j], f1r = Br[i][j + 1], f1i = Bi[i][j + 1];
+    T p00r, p01r, p00i, p01i, p10r, p11r, p10i, p11i, f0r, f1r, f0i, f1i;
+    lds_ld2(&B.r[j][j], p00r, p01r);
+    lds_ld2(&B.i[j][j], p00i, p01i);
+    lds_ld2(&B.r[j + 1][j], p10r, p11r);
+    lds_ld2(&B.i[j + 1][j], p10i, p11i);
+    lds_ld2(&B.r[i][j], f0r, f1r);
+    lds_ld2(&B.i[i][j], f0i, f1i);
     T h0r[4], h0i[4], h1r[4], h1i[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      h0r[q] = Br[j][c0 + q]; h0i[q] = Bi[j][c0 + q];
-      h1r[q] = Br[j + 1][c0 + q]; h1i[q] = Bi[j + 1][c0 + q];
-    }
-    // Q = P^-1 = adj(P') sc / det(P') with P' = sc P, sc = 1 / max |component|
-    // (det P itself would overflow float for Hz, whose entries reach ~1e20)
+    lds_ld4(&B.r[j][c0], h0r);
+    lds_ld4(&B.i[j][c0], h0i);
+    lds_ld4(&B.r[j + 1][c0], h1r);
+    lds_ld4(&B.i[j + 1][c0], h1i);
+    // Q = P^-1 = adj(P') sc / det(P') with P' = sc P, sc an exact power of two
+    // near 1 / max |component| (det P itself would overflow float for Hz,
+    // whose entries reach ~1e20)
     const T mx = fmax(fmax(fmax(fabs(p00r), fabs(p00i)), fmax(fabs(p01r), fabs(p01i))),
                       fmax(fmax(fabs(p10r), fabs(p10i)), fmax(fabs(p11r), fabs(p11i))));
-    const T sc = rcp_nr(mx);
+    const T sc = exp_scale(mx);
     const T a00r = p00r * sc, a00i = p00i * sc, a01r = p01r * sc, a01i = p01i * sc;
     const T a10r = p10r * sc, a10i = p10i * sc, a11r = p11r * sc, a11i = p11i * sc;
     const T dr = (a00r * a11r - a00i * a11i) - (a01r * a10r - a01i * a10i);
     const T di = (a00r * a11i + a00i * a11r) - (a01r * a10i + a01i * a10r);
-    if (!(fabs(dr) + fabs(di) > T(0)) || !isfinite(dr) || !isfinite(di) || !isfinite(sc)) bad = true;
-    T er, ei;
-    fast_crecip<T>(dr, di, er, ei);
-    er *= sc;
-    ei *= sc;
+    if (!(fabs(dr) + fabs(di) > T(0)) || !isfinite(dr) || !isfinite(di) || !(mx > T(0)) || !isfinite(mx))
+      bad = true;
+    // 1 / det: (dr - i di) / |det|^2 with det brought to [0.5, 1) by an exact scale
+    const T ds = exp_scale(fmax(fabs(dr), fabs(di)));
+    const T br = dr * ds, bi = di * ds;
+    const T rm = rcp_nr(br * br + bi * bi) * ds * sc;
+    const T er = br * rm, ei = -bi * rm;
     const T q00r = a11r * er - a11i * ei, q00i = a11r * ei + a11i * er;
     const T q01r = -(a01r * er - a01i * ei), q01i = -(a01r * ei + a01i * er);
     const T q10r = -(a10r * er - a10i * ei), q10i = -(a10r * ei + a10i * er);
@@ -549,14 +598,15 @@ __device__ void tile_inverse_diag(Lds<T> &L, int *status) {
     const T ckr = r0 ? q01r : (r1 ? q11r : -g1r), cki = r0 ? q01i : (r1 ? q11i : -g1i);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const T br = prow ? T(0) : ar[q], bi = prow ? T(0) : ai[q];
-      const T xr = br - (g0r * h0r[q] - g0i * h0i[q]) - (g1r * h1r[q] - g1i * h1i[q]);
-      const T xi = bi - (g0r * h0i[q] + g0i * h0r[q]) - (g1r * h1i[q] + g1i * h1r[q]);
+      const T xr0 = prow ? T(0) : ar[q], xi0 = prow ? T(0) : ai[q];
+      const T xr = xr0 - (g0r * h0r[q] - g0i * h0i[q]) - (g1r * h1r[q] - g1i * h1i[q]);
+      const T xi = xi0 - (g0r * h0i[q] + g0i * h0r[q]) - (g1r * h1i[q] + g1i * h1r[q]);
       const int c = c0 + q;
       ar[q] = c == j ? cjr : (c == j + 1 ? ckr : xr);
       ai[q] = c == j ? cji : (c == j + 1 ? cki : xi);
     }
   }
+  __syncthreads();   // the last step's reads are done before Ar, Ai are written
 #pragma unroll
   for (int q = 0; q < 4; ++q) { L.Ar[i][c0 + q] = ar[q]; L.Ai[i][c0 + q] = ai[q]; }
   if (bad) *status = 1;
@@ -883,7 +933,11 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
     L.Ai[a][b] = ci[t];
   }
   __syncthreads();
-  {
+  const bool next_pivot = (p + 1 < nt) && ti == p + 1 && tj == p + 1;
+  const bool next_block = (p == nt - 1) && ti == 0 && tj == 0 && k + 1 < J.m;
+  // the tile of the next pivot is replaced by its inverse at the next step
+  // and read by no workgroup before that: its store is skipped
+  if (!next_pivot) {
     const int i = tid >> 3, c0 = (tid & 7) * 4;
     const size_t idx = (size_t)(ti * TB + i) * J.np + tj * TB + c0;
     store4_nt(dst + idx, &L.Ar[i][c0]);
@@ -904,8 +958,6 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
       store4_nt(dst + np2 + mdx, mi);
     }
   }
-  const bool next_pivot = (p + 1 < nt) && ti == p + 1 && tj == p + 1;
-  const bool next_block = (p == nt - 1) && ti == 0 && tj == 0 && k + 1 < J.m;
   if (next_pivot || next_block) {
     // the tile is in Ar/Ai; S_{k+1}[0,0] overwrites it only after every
     // thread has read its row for the store above
@@ -1246,24 +1298,30 @@ __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const
   }
 }
 
-// Persistent sweeps: one launch per direction. Workgroup w owns rows
+// Persistent sweeps: one launch per direction, in two variants that differ
+// only in how the next block's vector passes between workgroups (the
+// tagged-word sweep for one right-hand side per pass, the flag sweep for
+// more). Workgroup w owns rows
 // [w*4*SRW, (w+1)*4*SRW) of every block; lane l of a wave covers columns
-// l, l+64, ... of its rows. Blocks are chained through per-workgroup flags:
-// step s waits until all W workgroups have finished step s-1. Each workgroup
-// computes its rows of y_k (forward) or d_k (backward) and, from them, its
-// rows of the next block's input vector (v_{k+1} = r_{k+1} - XM_{k+1} y_k
-// forward, w_{k-1} = XP_{k-1} d_k backward), which it publishes; the next
-// step then only loads that vector, with no per-workgroup recomputation.
-// Rows of G_k and the inputs of the published entries are loaded before the
-// wait, so HBM latency overlaps the synchronization. A wait that exceeds
-// SPIN_CAP polls raises the abort flag; every later wait then returns at
-// once, so the grid always drains.
-// The published vectors (two buffers alternating by step parity) are written
-// and read with agent-scope relaxed atomics, which are coherent across the
-// XCDs' L2 caches by themselves; the flags need only workgroup-scope ordering
-// around them, so no step writes back or invalidates a whole L2. A buffer is
-// rewritten two steps after it was read, and every reader has passed a flag
-// since. The row sums of a wave
+// l, l+64, ... of its rows. Each workgroup computes its rows of y_k (forward)
+// or d_k (backward) and, from them, its rows of the next block's input vector
+// (v_{k+1} = r_{k+1} - XM_{k+1} y_k forward, w_{k-1} = XP_{k-1} d_k backward),
+// which it publishes; every workgroup then loads the whole vector for the
+// next block. Rows of G_k and the inputs of the published entries are loaded
+// before the wait, so HBM latency overlaps the synchronization.
+// The vector is published as self-validating 64-bit words: 32 bits of the
+// value (complex64: one component; FP64: the low or the high half of one)
+// and a 32-bit tag naming the step it feeds, written and read with
+// agent-scope relaxed atomics (one 64-bit access each, coherent across the
+// XCDs' L2 caches). A consumer polls its words until every tag shows the
+// step; no flag, no store-completion wait and no second round trip per
+// block. Tags grow across the launches of one solve (the host passes each
+// launch's base), so a word left by an earlier launch never matches. Two
+// buffers alternate by step parity: a workgroup writes a buffer at step s
+// only after receiving every entry of step s's input, i.e. after every
+// workgroup has finished reading that buffer at step s - 1. A wait that
+// exceeds the spin cap raises the abort flag; every later wait then returns
+// at once, so the grid always drains. The row sums of a wave
 // (SRW rows x RP right-hand sides x re/im) are reduced together. RP, the
 // right-hand sides of one pass rounded up to a power of two, is a template
 // parameter so the unrolled loops stay small enough for the instruction
@@ -1275,17 +1333,22 @@ constexpr int MAXCL = PSWEEP_MAX_N / 64;   // columns per lane at that block
 constexpr int MAXRHS = 8;           // right-hand sides per pass (complex64)
 // right-hand sides per pass: FP64 holds half as many row sums in registers
 template <typename T> constexpr int sweep_rhs() { return sizeof(T) == 8 ? MAXRHS / 2 : MAXRHS; }
+// tagged words per vector component: the value's 32-bit halves
+template <typename T> constexpr int sweep_wpc() { return sizeof(T) == 8 ? 2 : 1; }
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 inline int pad64(int n) { return (n + 63) / 64 * 64; }
 
-constexpr int MAXW = PSWEEP_MAX_N / (4 * SRW);   // sweep workgroups at that block
-
+typedef unsigned long long u64;
+constexpr int MAXW = PSWEEP_MAX_N / (4 * SRW);   // sweep workgroups at the largest block
+constexpr int LL_MAXRHS = 1;                      // right-hand sides of the tagged-word sweep
 struct SweepSync {
   unsigned abort;
   unsigned pad[63];
-  unsigned done[2][MAXW];   // forward / backward: steps finished per workgroup
-  // the next block's input vector, published row by row: [2 parity][RP][np] complex
+  unsigned done[2][MAXW];   // flag sweep, forward / backward: steps finished per workgroup
+  // flag sweep: the next block's input vector, [2 parity][RP][np] complex
   double vbuf[2 * MAXRHS * PSWEEP_MAX_N];
+  // tagged-word sweep: [2 parity][RP][np][re/im][words per component]
+  u64 llbuf[2 * LL_MAXRHS * PSWEEP_MAX_N * 2 * 2];
 };
 
 // Sums v[0..N) over the 64 lanes of a wave at once: each round halves the
@@ -1316,6 +1379,16 @@ __device__ inline T wave_multi_sum(T (&v)[N], int lane) {
   return t;
 }
 
+// publish one component of a vector entry with its tag
+__device__ inline void publish_tagged(u64 *w, float v, unsigned tag) {
+  st_xcd(w, (u64)__float_as_uint(v) | ((u64)tag << 32));
+}
+__device__ inline void publish_tagged(u64 *w, double v, unsigned tag) {
+  const u64 bits = (u64)__double_as_longlong(v);
+  st_xcd(w, (bits & 0xffffffffull) | ((u64)tag << 32));
+  st_xcd(w + 1, (bits >> 32) | ((u64)tag << 32));
+}
+
 // a complex entry of a published vector, one load per entry
 __device__ inline void ld_xcd_c(const float *p, float &re, float &im) {
   const unsigned long long v = ld_xcd(reinterpret_cast<const unsigned long long *>(p));
@@ -1327,6 +1400,14 @@ __device__ inline void ld_xcd_c(const double *p, double &re, double &im) {
   im = ld_xcd(p + 1);
 }
 
+// The flag sweep (two or more right-hand sides per pass): each workgroup
+// publishes its entries of the next block's vector with agent-scope relaxed
+// atomics, waits for its own stores to complete, passes a barrier and raises
+// its per-workgroup flag; consumers wait for all W flags of the previous
+// step, then load the vector. Per block that is three memory round trips,
+// but the vector moves as plain 8-byte values: with several right-hand sides
+// this is faster than the tagged words below, which double the bytes and
+// re-poll every word.
 template <typename T, int RP>
 __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const double *__restrict__ op,
                                                const double *__restrict__ r, T *y, double *__restrict__ x,
@@ -1476,6 +1557,205 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
     __syncthreads();
     if (tid == 0)
       __hip_atomic_store(&sy->done[dir][blockIdx.x], (unsigned)(s + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+
+template <typename T, int RP>
+__global__ void __launch_bounds__(NT) k_psweep_ll(const T *__restrict__ G, const double *__restrict__ op,
+                                                  const double *__restrict__ r, T *y, double *__restrict__ x,
+                                                  int nx, int ny, int np, int nrhs, int dir, SweepSync *sy,
+                                                  unsigned cap, unsigned tag0) {
+  extern __shared__ unsigned char smem_raw[];
+  const int nv = (np + 63) / 64 * 64;                  // padded row length of v
+  T *vr = reinterpret_cast<T *>(smem_raw);             // [RP][nv]
+  T *vi = vr + (size_t)RP * nv;
+  constexpr int WPC = sweep_wpc<T>();
+  const size_t vstride = (size_t)RP * np * 2 * WPC;    // words of one parity buffer
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const size_t np2 = (size_t)np * np, M = (size_t)nx * ny;
+  const int row0 = (blockIdx.x * 4 + wv) * SRW;
+  // the (row, rhs, re/im) value this lane finishes after the reduction
+  static_assert(RP >= 1 && RP <= MAXRHS && (RP & (RP - 1)) == 0, "RP must be a power of two");
+  constexpr int NV = 2 * SRW * RP, SH = 6 - ilog2(NV);
+  const int jv = lane >> SH;   // lanes jv << SH ... all hold value jv
+  const int my_ri = jv & 1, my_q = (jv >> 1) % RP, my_rr = jv / (2 * RP);
+  const int my_a = row0 + my_rr;
+  const bool my_store = (lane & ((1 << SH) - 1)) == 0 && my_q < nrhs && my_a < np;
+  const bool my_real = my_q < nrhs && my_a < ny;
+  const int cpl_plane = dir == 0 ? FDFD_OP_XM : FDFD_OP_XP;
+  // words this thread receives per step: component c of rhs q is word
+  // ((q * np + c) * 2 + re/im) * WPC + half
+  constexpr int CW = (RP * PSWEEP_MAX_N * 2 * WPC + NT - 1) / NT;
+  constexpr int CH = CW < 24 ? CW : 24;
+  static_assert(CW <= 64, "receive mask");
+  const int nword = nrhs * np * 2 * WPC;
+  // word tid + u * NT is component (re/im) w_ri, half w_half of entry c of
+  // rhs q; consecutive words of a thread are WSTEP entries apart
+  constexpr int WSTEP = NT / (2 * WPC);
+  const int w_half = tid % WPC, w_ri = (tid / WPC) & 1, w_qc = tid / (2 * WPC);
+  const int w_q0 = w_qc / np, w_c0 = w_qc % np;
+  T *const w_dst = w_ri == 0 ? vr : vi;
+  u64 expect = 0;
+#pragma unroll
+  for (int u = 0; u < CW; ++u)
+    if (tid + u * NT < nword) expect |= 1ull << u;
+  // rows of unused right-hand sides and the column padding stay zero
+  for (int e = tid; e < RP * nv; e += NT) {
+    vr[e] = T(0);
+    vi[e] = T(0);
+  }
+  for (int s = 0; s < nx; ++s) {
+    const int k = dir == 0 ? s : nx - 1 - s;
+    const T *Gk = G + (size_t)k * 2 * np2;
+    const bool coupled = dir == 0 ? k > 0 : k + 1 < nx;
+    const bool active = dir == 0 || coupled;
+    const int kn = dir == 0 ? k + 1 : k - 1;             // the next block
+    const bool has_next = dir == 0 ? k + 1 < nx : k > 0;
+    // 1. this workgroup's rows of G_k and the inputs of its published
+    //    entries (independent of the previous block)
+    T gr[SRW][MAXCL], gi[SRW][MAXCL];
+#pragma unroll
+    for (int rr = 0; rr < SRW; ++rr) {
+      const int a = row0 + rr;
+#pragma unroll
+      for (int cc = 0; cc < MAXCL; ++cc) {
+        const int c = lane + 64 * cc;
+        const bool ok = a < np && c < np && active;
+        gr[rr][cc] = ok ? Gk[(size_t)a * np + c] : T(0);
+        gi[rr][cc] = ok ? Gk[np2 + (size_t)a * np + c] : T(0);
+      }
+    }
+    cd rn = {0, 0}, cn = {0, 0};   // forward: r_{k+1}, XM_{k+1}; backward: XP_{k-1} (this lane's row)
+    if (my_real && has_next) {
+      if (dir == 0) rn = ld_c(r, (size_t)my_q * M + (size_t)kn * ny + my_a);
+      cn = ld_c(op, cpl_plane * M + (size_t)kn * ny + my_a);
+    }
+    T yown = 0;   // backward: this lane's entry of y_k, written by the forward launch
+    T *my_y = y + 2 * (((size_t)my_q * nx + k) * np + my_a) + my_ri;
+    if (dir == 1 && my_store) yown = *my_y;
+    // 2. this block's input vector into LDS: r_0 at the first forward step,
+    //    zero at the first backward step, else the vector all workgroups
+    //    published at the previous step
+    __syncthreads();   // every wave has finished reading the previous block's vector
+    if (s == 0) {
+      constexpr int CU = (PSWEEP_MAX_N + NT - 1) / NT;
+      double ar[RP][CU], ai[RP][CU];
+#pragma unroll
+      for (int q = 0; q < RP; ++q)
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+          const int c = tid + u * NT;
+          const bool ok = dir == 0 && q < nrhs && c < ny;
+          const double *rp = r + 2 * ((size_t)q * M + (size_t)k * ny + c);
+          ar[q][u] = ok ? rp[0] : 0.0;
+          ai[q][u] = ok ? rp[1] : 0.0;
+        }
+#pragma unroll
+      for (int q = 0; q < RP; ++q)
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+          const int c = tid + u * NT;
+          if (c < nv) {
+            vr[q * nv + c] = (T)ar[q][u];
+            vi[q * nv + c] = (T)ai[q][u];
+          }
+        }
+    } else {
+      const u64 *vin = sy->llbuf + (size_t)(s & 1) * vstride;
+      const unsigned tag = tag0 + (unsigned)s;
+      unsigned spins = 0;
+      bool give_up = false;
+      int qa = w_q0, ca = w_c0;   // (rhs, entry) of the chunk's first word
+      // words in chunks of at most CH loads in flight (bounds the registers)
+#pragma unroll
+      for (int u0 = 0; u0 < CW; u0 += CH) {
+        u64 pending = (expect >> u0) & ((CH >= 64) ? ~0ull : ((1ull << CH) - 1));
+        // many words: wait on one of them first, so the spin reads one word
+        // per thread instead of all of them
+        if (__popcll(pending) > 16) {
+          const u64 *first = vin + tid + (u0 + __builtin_ctzll(pending)) * NT;
+          while (!give_up && (unsigned)(ld_xcd(first) >> 32) != tag) {
+            if ((++spins & 63) == 0 && ld_xcd(&sy->abort)) give_up = true;
+            if (spins > cap) {
+              st_xcd(&sy->abort, 1u);
+              give_up = true;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+        }
+        while (pending && !give_up) {
+          u64 w[CH];
+#pragma unroll
+          for (int u = 0; u < CH; ++u)
+            w[u] = (u0 + u < CW && ((pending >> u) & 1)) ? ld_xcd(vin + tid + (u0 + u) * NT) : 0ull;
+          int q = qa, c = ca;
+#pragma unroll
+          for (int u = 0; u < CH; ++u) {
+            if (u0 + u < CW && ((pending >> u) & 1) && (unsigned)(w[u] >> 32) == tag) {
+              pending &= ~(1ull << u);
+              const unsigned bits = (unsigned)(w[u] & 0xffffffffull);
+              __builtin_memcpy(reinterpret_cast<unsigned char *>(w_dst + q * nv + c) + 4 * w_half, &bits, 4);
+            }
+            c += WSTEP;
+            while (c >= np) { c -= np; ++q; }
+          }
+          if (!pending) break;
+          if ((++spins & 63) == 0 && ld_xcd(&sy->abort)) give_up = true;
+          if (spins > cap) {
+            st_xcd(&sy->abort, 1u);
+            give_up = true;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {   // advance to the next chunk's first word
+          ca += WSTEP;
+          while (ca >= np) { ca -= np; ++qa; }
+        }
+      }
+    }
+    __syncthreads();
+    // 3. partial row dot products for every (row, rhs) pair of this wave
+    T acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = 0;
+    if (active) {
+#pragma unroll
+      for (int cc = 0; cc < MAXCL; ++cc) {
+        if (64 * cc < np) {   // uniform; columns c >= np hold G = 0 and v = 0
+          const int c = lane + 64 * cc;
+#pragma unroll
+          for (int q = 0; q < RP; ++q) {
+            const T br = vr[q * nv + c], bi = vi[q * nv + c];
+#pragma unroll
+            for (int rr = 0; rr < SRW; ++rr) {
+              T &sr = acc[(rr * RP + q) * 2], &si = acc[(rr * RP + q) * 2 + 1];
+              sr += gr[rr][cc] * br - gi[rr][cc] * bi;
+              si += gr[rr][cc] * bi + gi[rr][cc] * br;
+            }
+          }
+        }
+      }
+    }
+    const T sum = wave_multi_sum(acc, lane);
+    // 4. this lane's entry of y_k (forward) or d_k (backward)
+    const T out = dir == 0 ? sum : yown - sum;
+    if (my_store) {
+      if (dir == 0) *my_y = out;   // read by the backward launch
+      else if (my_a < ny) x[2 * ((size_t)my_q * M + (size_t)k * ny + my_a) + my_ri] += (double)out;
+    }
+    // 5. publish the next block's input entry for this row:
+    //    forward v_{k+1} = r_{k+1} - XM_{k+1} y_k, backward w_{k-1} = XP_{k-1} d_k
+    const T other = __shfl_xor(out, 1 << SH);   // the other component of this (row, rhs)
+    if (my_store && has_next) {
+      const double ore = (double)(my_ri == 0 ? out : other), oim = (double)(my_ri == 0 ? other : out);
+      const cd t = cmul(cn, cd{ore, oim});
+      const cd v = dir == 0 ? csub(rn, t) : t;
+      u64 *vout = sy->llbuf + (size_t)((s + 1) & 1) * vstride;
+      publish_tagged(vout + (((size_t)my_q * np + my_a) * 2 + my_ri) * WPC, (T)(my_ri == 0 ? v.r : v.i),
+                     tag0 + (unsigned)(s + 1));
+    }
   }
 }
 
@@ -1998,6 +2278,7 @@ size_t factor_work_bytes(const fdfd_grid *g, int precision) {
 
 struct SolveWork {
   double *r; void *y; double *part; double *norms; SweepSync *sync;
+  unsigned tag_next;     // tag base of the next persistent sweep launch (host side)
   double *t, *f;         // wrap correction: T^-1 r and -U C K^-1 V^T t
   double *u, *w1, *w2;   // truncated wrap: [nrhs][n] complex each
 };
@@ -2005,6 +2286,7 @@ SolveWork solve_work_layout(const fdfd_grid *g, int precision, int nrhs, void *w
   const size_t M = (size_t)g->nx * g->ny, np = pad_n(g->ny), es = elem_size(precision);
   char *base = (char *)work;
   SolveWork sw;
+  sw.tag_next = 1;
   size_t off = 0;
   sw.r = (double *)(base + off); off += align_up(16 * M * nrhs, 256);
   sw.y = base + off;             off += align_up(2 * es * (size_t)g->nx * np * nrhs, 256);
@@ -2322,21 +2604,29 @@ int wrap_apply_truncated(const fdfd_grid *g, const double *op, const void *facto
 
 template <typename T, int RP>
 void launch_psweep(int W, size_t lds, hipStream_t s, const T *G, const double *op, const double *r, T *y,
-                   double *x, int nx, int ny, int np, int nr, int dir, SweepSync *sy) {
+                   double *x, int nx, int ny, int np, int nr, int dir, SweepSync *sy, unsigned tag0) {
   static std::once_flag lds_attr;
   std::call_once(lds_attr, [] {
-    (void)hipFuncSetAttribute((const void *)k_psweep<T, RP>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(2 * RP * PSWEEP_MAX_N * sizeof(T)));
+    if constexpr (RP <= LL_MAXRHS)
+      (void)hipFuncSetAttribute((const void *)k_psweep_ll<T, RP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(2 * RP * PSWEEP_MAX_N * sizeof(T)));
+    else
+      (void)hipFuncSetAttribute((const void *)k_psweep<T, RP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(2 * RP * PSWEEP_MAX_N * sizeof(T)));
   });
-  hipLaunchKernelGGL((k_psweep<T, RP>), dim3(W), dim3(NT), lds, s, G, op, r, y, x, nx, ny, np, nr, dir, sy,
-                     g_spin_cap.load());
+  if constexpr (RP <= LL_MAXRHS)
+    hipLaunchKernelGGL((k_psweep_ll<T, RP>), dim3(W), dim3(NT), lds, s, G, op, r, y, x, nx, ny, np, nr, dir, sy,
+                       g_spin_cap.load(), tag0);
+  else
+    hipLaunchKernelGGL((k_psweep<T, RP>), dim3(W), dim3(NT), lds, s, G, op, r, y, x, nx, ny, np, nr, dir, sy,
+                       g_spin_cap.load());
 }
 
 // x += F^-1 r with the persistent sweep kernels (RHS in passes of
 // sweep_rhs<T>()); blockwise = true uses one launch per block instead.
 template <typename T>
 int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const double *r,
-               double *x, int nrhs, const SolveWork &sw, hipStream_t s, bool blockwise) {
+               double *x, int nrhs, SolveWork &sw, hipStream_t s, bool blockwise) {
   // blocks beyond the persistent kernel's register tiles: one launch per block
   if (blockwise || pad_n(g->ny) > PSWEEP_MAX_N) return sweep_blockwise<T>(g, op, factor, r, x, nrhs, sw.y, s);
   const int nx = g->nx, ny = g->ny, np = pad_n(ny);
@@ -2347,20 +2637,28 @@ int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const d
     const int rp = nr <= 1 ? 1 : nr <= 2 ? 2 : nr <= 4 ? 4 : 8;
     T *yq = (T *)sw.y + (size_t)2 * q0 * nx * np;
     const size_t lds = (size_t)2 * rp * pad64(np) * sizeof(T);
-    // step flags start at zero; the abort flag is cleared by the caller, so
-    // an abort stays visible across the sweeps of one solve attempt
-    HIP_TRY(hipMemsetAsync(sw.sync->done, 0, sizeof sw.sync->done, s));
+    // the abort flag is cleared by the caller, so an abort stays visible
+    // across the sweeps of one solve attempt. Flag sweep: the step flags
+    // start at zero. Tagged-word sweep: each launch's tags start past every
+    // tag written before in this workspace since it was cleared.
+    if (rp > LL_MAXRHS) HIP_TRY(hipMemsetAsync(sw.sync->done, 0, sizeof sw.sync->done, s));
     for (int dir = 0; dir < 2; ++dir) {
+      if (sw.tag_next > 0xffffffffu - (unsigned)nx - 2) {
+        HIP_TRY(hipMemsetAsync(sw.sync->llbuf, 0, sizeof sw.sync->llbuf, s));
+        sw.tag_next = 1;
+      }
+      const unsigned tag0 = sw.tag_next;
+      sw.tag_next += (unsigned)nx + 1;
       const double *rq = dir == 0 ? r + 2 * M * q0 : nullptr;
       double *xq = dir == 1 ? x + 2 * M * q0 : nullptr;
       const T *G = (const T *)factor;
       switch (rp) {
-        case 1: launch_psweep<T, 1>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync); break;
-        case 2: launch_psweep<T, 2>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync); break;
-        case 4: launch_psweep<T, 4>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync); break;
+        case 1: launch_psweep<T, 1>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync, tag0); break;
+        case 2: launch_psweep<T, 2>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync, tag0); break;
+        case 4: launch_psweep<T, 4>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync, tag0); break;
         default:
           if constexpr (sweep_rhs<T>() >= 8)
-            launch_psweep<T, 8>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync);
+            launch_psweep<T, 8>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync, tag0);
           break;
       }
     }
@@ -2740,22 +3038,25 @@ int fdfd_stretch_scale(const fdfd_grid *g, double *v, int nrhs, int inverse, hip
   return FDFD_OK;
 }
 
-int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int precision,
-               const double *b, double *x, int nrhs, int transpose, double tol, int max_iter,
-               void *work, fdfd_solve_stats *stats, double *rel_res, hipStream_t stream) {
-  if (int rc = check_grid(g)) return rc;
-  if (int rc = check_prec(precision)) return rc;
-  if (!op || !factor || !b || !x || !work || nrhs < 1 || max_iter < 1 || !(tol > 0))
-    return fail(FDFD_ERR_ARG, "bad arguments to fdfd_solve");
+}  // extern "C"
+
+namespace {
+// The refinement of fdfd_solve. work: the solve workspace (device); host:
+// 4 * nrhs doubles of host scratch (norms of b, of the residual, the relative
+// residuals and the previous step's).
+int solve_impl(const fdfd_grid *g, const double *op, const void *factor, int precision, const double *b,
+               double *x, int nrhs, int transpose, double tol, int max_iter, void *work,
+               fdfd_solve_stats *stats, double *rel_res, hipStream_t stream, double *host) {
   SolveWork sw = solve_work_layout(g, precision, nrhs, work);
   const size_t M = (size_t)g->nx * g->ny;
   const dim3 sgrid((g->ny + NT - 1) / NT, g->nx, nrhs);
   fdfd_solve_stats st;
   memset(&st, 0, sizeof st);
-  std::vector<double> nb(nrhs), nr(nrhs), rel(nrhs, 0.0);
+  double *nb = host, *nr = host + nrhs, *rel = host + 2 * nrhs, *prev = host + 3 * nrhs;
+  for (int q = 0; q < nrhs; ++q) rel[q] = 0.0;
   // ||b|| in the caller's system
   if (int rc = norms_async(g, b, nrhs, 0, sw.part, sw.norms, stream)) return rc;
-  HIP_TRY(hipMemcpyAsync(nb.data(), sw.norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipMemcpyAsync(nb, sw.norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   bool all_zero = true;
   for (int q = 0; q < nrhs; ++q) all_zero = all_zero && nb[q] == 0.0;
@@ -2778,7 +3079,7 @@ int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int pre
     HIP_TRY(hipMemsetAsync(x, 0, 16 * M * nrhs, stream));
     HIP_TRY(hipMemcpyAsync(rr, b, 16 * M * nrhs, hipMemcpyDeviceToDevice, stream));
     if (transpose && (rc = fdfd_stretch_scale(g, rr, nrhs, 1, stream))) return rc;
-    std::vector<double> prev(nrhs, INFINITY);
+    for (int q = 0; q < nrhs; ++q) prev[q] = INFINITY;
     int stall_count = 0;
     bool aborted = false;
     const bool fp64 = base_prec(precision) == FDFD_PREC_FP64;
@@ -2814,7 +3115,7 @@ int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int pre
       // transposed: ||b - A^T x|| = ||D (b' - A x')||
       if ((rc = norms_async(g, rr, nrhs, transpose, sw.part, sw.norms, stream))) return rc;
       unsigned ab = 0;
-      HIP_TRY(hipMemcpyAsync(nr.data(), sw.norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipMemcpyAsync(nr, sw.norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, stream));
       HIP_TRY(hipMemcpyAsync(&ab, &sw.sync->abort, sizeof ab, hipMemcpyDeviceToHost, stream));
       HIP_TRY(hipStreamSynchronize(stream));
       if (ab && !blockwise) { aborted = true; break; }
@@ -2847,6 +3148,22 @@ int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int pre
     return fail(FDFD_ERR_NOT_CONVERGED, "refinement reached %.3e after %d sweeps (tol %.1e)%s",
                 st.max_rel_residual, st.iterations, tol, st.stalled ? ", stalled" : "");
   return FDFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int precision,
+               const double *b, double *x, int nrhs, int transpose, double tol, int max_iter,
+               void *work, fdfd_solve_stats *stats, double *rel_res, hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (int rc = check_prec(precision)) return rc;
+  if (!op || !factor || !b || !x || !work || nrhs < 1 || max_iter < 1 || !(tol > 0))
+    return fail(FDFD_ERR_ARG, "bad arguments to fdfd_solve");
+  std::vector<double> host(4 * (size_t)nrhs);
+  return solve_impl(g, op, factor, precision, b, x, nrhs, transpose, tol, max_iter, work, stats, rel_res,
+                    stream, host.data());
 }
 
 int fdfd_grad_eps(const fdfd_grid *g, const double *lam, const double *e, double *grad,

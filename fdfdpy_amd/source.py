@@ -66,31 +66,33 @@ class mode:
         (mode.py:26-60). The waveguide keeps the cells of the source line
         whose eps exceeds 1, at the largest |eps| of the domain; the copy of
         the simulation keeps its source, and the mode is added to it."""
-        original_eps = _host(simulation.eps_r)
-        Nx, Ny = original_eps.shape
-        eps_max = np.max(np.abs(original_eps))
-        norm_eps = np.ones((Nx, Ny))
-        new_center = list(self.center)
+        eps_host = _host(simulation.eps_r)
+        Nx, Ny = eps_host.shape
+        core_eps = np.max(np.abs(eps_host))
+        guide = np.ones((Nx, Ny))
+        probe_at = list(self.center)
         if self.direction_normal == "x":
-            norm_eps[:, original_eps[self.center[0], :] > 1] = eps_max
-            new_center[0] = Nx - new_center[0]
+            # a guide along x: the source line's core cells, extended over every column
+            guide[:, eps_host[self.center[0], :] > 1] = core_eps
+            probe_at[0] = Nx - probe_at[0]
         elif self.direction_normal == "y":
-            norm_eps[original_eps[:, self.center[1]] > 1, :] = eps_max
-            new_center[1] = Ny - new_center[1]
+            guide[eps_host[:, self.center[1]] > 1, :] = core_eps
+            probe_at[1] = Ny - probe_at[1]
         else:
             raise ValueError("The value of direction_normal is not x or y!")
 
-        norm = type(simulation)(simulation.omega, norm_eps, simulation.dl, simulation.NPML, simulation.pol,
-                                simulation.L0, device=simulation.device)
+        straight = type(simulation)(simulation.omega, guide, simulation.dl, simulation.NPML, simulation.pol,
+                                    simulation.L0, device=simulation.device)
         src = simulation.src
-        norm.src = src.clone() if isinstance(src, torch.Tensor) else np.array(src, copy=True)
-        self.insert_mode(norm, norm.src)
-        norm.solve_fields()
-        simulation.W_in = norm.flux_probe(self.direction_normal, new_center, self.width)
+        straight.src = src.clone() if isinstance(src, torch.Tensor) else np.array(src, copy=True)
+        self.insert_mode(straight, straight.src)
+        straight.solve_fields()
+        # the flux through the mirrored line at the far end of the guide
+        simulation.W_in = straight.flux_probe(self.direction_normal, probe_at, self.width)
         # mode.py:59-60 reads 'Ez' whatever the polarization (a Hz mode has
         # no such field there either)
-        Ez = _host(norm.fields['Ez'])
-        simulation.E2_in = np.sum(np.square(np.abs(Ez)) * np.abs(_host(norm.src)))
+        e_line = _host(straight.fields['Ez'])
+        simulation.E2_in = np.sum(np.square(np.abs(e_line)) * np.abs(_host(straight.src)))
 
     def insert_mode(self, simulation, destination):
         """Writes the scaled mode profile into ``destination`` on the source

@@ -55,11 +55,11 @@ def test_c64_stall_falls_back_to_fp64():
     assert np.linalg.norm(xh - xr) / np.linalg.norm(xr) <= 1e-10
 
 
-def test_singular_pivot_block_falls_back_to_pivoted_factor():
+def test_singular_pivot_block_leaves_complex64_for_fp64():
     """eps[0, 0] is set so that the leading 2 x 2 block of the first pivot
-    tile is singular (to rounding) while A is not: the diagonal-pivot factors
-    (complex64, then FP64) are unusable, and the FP64 factor with partial
-    pivoting inside the tiles solves it."""
+    tile is singular to rounding while A is not. The complex64 factor cannot
+    refine it; the FP64 factor, whose 2 x 2 pivot amplifies rounding by about
+    1e13 (still below 1 / eps), refines to the tolerance."""
     omega, eps, src, npml = _problem()
     Nx, Ny = eps.shape
     A, _ = construct_A(omega, [0, Nx * 0.02], [0, Ny * 0.02], eps, npml, 'Ez', 1e-6)
@@ -70,12 +70,41 @@ def test_singular_pivot_block_falls_back_to_pivoted_factor():
     A2 = A2.tocsr()
     det = A2[0, 0] * A2[1, 1] - A2[0, 1] * A2[1, 0]
     assert abs(det) < 1e-12 * abs(A2[0, 0] * A2[1, 1])     # the planted singular minor
-    x, rep = _device_solve(omega, eps, src, npml, precision='c64', tol=1e-10)
+    x, rep = _device_solve(omega, eps, src, npml, precision='c64', tol=1e-12)
     print(f"singular 2x2 pivot: solved by {rep.precision} pivoted={rep.pivoted} in {rep.iterations} steps; "
           f"factors with a bad pivot: {rep.failed_factors}")
-    assert rep.converged and rep.fell_back and rep.pivoted and rep.precision == 'fp64'
+    assert rep.converged and rep.fell_back and rep.precision == 'fp64'
     xh = x.cpu().numpy()
     assert np.linalg.norm(xh - xr) / np.linalg.norm(xr) <= 1e-10
+
+
+def test_zero_pivot_falls_back_to_pivoted_factor(monkeypatch):
+    """The last step of the order: the diagonal-pivot factors report a zero
+    or non-finite pivot (FDFD_ERR_NUMERIC, forced here from the host for
+    every unpivoted factor), and the FP64 factor with partial pivoting inside
+    the tiles solves the system to the oracle's field."""
+    omega, eps, src, npml = _problem()
+    real_factor = DeviceOperator.factor
+
+    def factor(self, precision="fp64", refresh=False, pivot=False, stepwise=False):
+        resident_pivoted = precision in self.factors and self.factors[precision][2]
+        if not pivot and not resident_pivoted:     # building an unpivoted factor
+            self.factors.pop(precision, None)
+            raise L.FdfdError(L.FDFD_ERR_NUMERIC, "zero or non-finite pivot in the block factorization")
+        return real_factor(self, precision, refresh=refresh, pivot=pivot, stepwise=stepwise)
+    monkeypatch.setattr(DeviceOperator, "factor", factor)
+    x, rep = _device_solve(omega, eps, src, npml, precision='c64', tol=1e-12)
+    print(f"forced zero pivots: solved by {rep.precision} pivoted={rep.pivoted}, failed {rep.failed_factors}")
+    assert rep.converged and rep.fell_back and rep.pivoted and rep.precision == 'fp64'
+    assert rep.failed_factors == [('c64', False), ('fp64', False)]
+    _, xr = _oracle_x(omega, eps, src, npml)
+    xh = x.cpu().numpy()
+    assert np.linalg.norm(xh - xr) / np.linalg.norm(xr) <= 1e-10
+    # every factor failing is an error of the solve, not a silent field
+    monkeypatch.setattr(DeviceOperator, "factor", lambda self, *a, **k: (_ for _ in ()).throw(
+        L.FdfdError(L.FDFD_ERR_NUMERIC, "zero pivot")))
+    with pytest.raises(L.FdfdError):
+        _device_solve(omega, eps, src, npml, precision='c64', tol=1e-10)
 
 
 def test_no_factor_converges_raises():

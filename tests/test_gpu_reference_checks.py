@@ -1,18 +1,30 @@
 """The reference's own result checks, run on the device path.
 
-- tests/test_flux.py:11-37: the flux of a modal source through a probe line
-  does not depend on the mesh (300 x 100 at dl = 0.01 against 600 x 200 at
-  dl = 0.005), rtol 1e-3.
+- tests/test_flux.py:11-37 compares the raw flux of a modal source at
+  300 x 100 (dl = 0.01) with 600 x 200 (dl = 0.005) at rtol 1e-3. The
+  reference's mode source is the unit-norm eigenvector of the line
+  (source/mode.py:96-107, no dl factor), so its raw flux scales with the mesh
+  (dl^3: the two fluxes differ by 7.97x in the reference's algorithm as
+  restated by the oracle, which the device reproduces to 1e-9 at both
+  meshes). The mesh-independent quantity is the flux normalized by the input
+  power W_in that the mode source computes for this purpose: it agrees
+  between the meshes at the reference's rtol 1e-3.
 - tests/test_nonlinear_solvers.py:12-58: Newton's method and the Born
   iteration reach the same Kerr field with eps_scale=True (the branch of
   nonlinearity.py:19-26). The reference compares ``fields['Ez']``, which
   neither solver writes (both fill ``fields_nl``), so it compares the linear
   field with itself; here the nonlinear fields are compared.
-- tests/test_chi3.py:9-79: the self-phase modulation of a waveguide mode
-  against the analytic nonlinear phase gamma P L. The reference test calls a
-  solve_fields_nl signature that no longer exists and reads the linear
-  fields; here the Kerr term is added with add_nl (chi3 / L0^2, simulation.py:
-  70-73) and the phase and power are read from the nonlinear fields.
+- tests/test_chi3.py:9-79: the self-phase modulation of a waveguide mode.
+  The reference test calls a solve_fields_nl signature that no longer exists,
+  reads the linear fields, and its analytic n2 (test_chi3.py:63-64) is off by
+  a units factor (the device phase is 12.6 times it, in the reference's
+  restated algorithm as well). Here the Kerr term is added with add_nl
+  (chi3 / L0^2, simulation.py:70-73), the phase is read from the nonlinear
+  fields, and it is checked against two predictions that do not run the
+  nonlinear solver: a linear solve with eps_r raised by 3 chi3 / L0^2 |E|^2
+  of the linear field (first order in the Kerr term), and first-order
+  perturbation theory of the guided mode (the 1D eigenproblem of the
+  waveguide's cross-section).
 """
 
 import numpy as np
@@ -24,23 +36,30 @@ pytestmark = pytest.mark.gpu
 
 
 def test_flux_resolution_consistency():
+    from oracle import OracleSimulation
     omega = 2 * np.pi * 200e12
-    eps1 = np.ones((300, 100))
-    eps1[:, 40:60] = 12.25
-    s1 = Simulation(omega, eps1, 0.01, [15, 15], 'Ez')
-    s1.add_mode(3.5, 'x', [20, 50], 60, scale=1)
-    s1.setup_modes()
-    s1.solve_fields()
-    flux1 = s1.flux_probe('x', [150, 50], 60)
-    eps2 = np.ones((600, 200))
-    eps2[:, 80:120] = 12.25
-    s2 = Simulation(omega, eps2, 0.005, [15, 15], 'Ez')
-    s2.add_mode(3.5, 'x', [20, 100], 120, scale=1)
-    s2.setup_modes()
-    s2.solve_fields()
-    flux2 = s2.flux_probe('x', [300, 100], 120)
-    print(f"flux 300x100 {flux1:.9e}, 600x200 {flux2:.9e}, relative difference {abs(flux1 - flux2) / abs(flux2):.2e}")
-    np.testing.assert_allclose(flux1, flux2, rtol=1e-3)
+    cases = []
+    for (Nx, Ny, dl, core, center, width, probe) in ((300, 100, 0.01, (40, 60), [20, 50], 60, [150, 50]),
+                                                     (600, 200, 0.005, (80, 120), [20, 100], 120, [300, 100])):
+        eps = np.ones((Nx, Ny))
+        eps[:, core[0]:core[1]] = 12.25
+        sim = Simulation(omega, eps, dl, [15, 15], 'Ez')
+        sim.add_mode(3.5, 'x', center, width, scale=1)
+        sim.setup_modes()
+        sim.solve_fields()
+        flux = sim.flux_probe('x', probe, width)
+        ref = OracleSimulation(omega, eps, dl, [15, 15], 'Ez')
+        ref.add_mode(3.5, 'x', center, width, scale=1)
+        ref.setup_modes()
+        ref.solve_fields()
+        rflux = ref.flux_probe('x', probe, width)
+        assert abs(flux - rflux) <= 1e-9 * abs(rflux)
+        assert abs(sim.W_in - ref.W_in) <= 1e-9 * abs(ref.W_in)
+        cases.append((flux, sim.W_in))
+    (f1, w1), (f2, w2) = cases
+    print(f"flux {f1:.9e} / {f2:.9e} (ratio {f1 / f2:.4f}); flux / W_in {f1 / w1:.6f} / {f2 / w2:.6f}")
+    assert 7.5 < f1 / f2 < 8.5          # the unit-norm line source: raw flux ~ dl^3
+    np.testing.assert_allclose(f1 / w1, f2 / w2, rtol=1e-3)
 
 
 def _kerr_waveguide(eps_scale):
@@ -82,30 +101,65 @@ def test_newton_equals_born_with_eps_scale():
         assert diff <= 1e-8
 
 
+def _mode_phase_per_amp2(sim, E, region, L):
+    """First-order perturbation theory of the guided mode: the phase (in
+    units of pi) that eps_r + 3 chi3/L0^2 |E|^2 adds over the Kerr region, per
+    unit source amplitude squared, from the 1D eigenproblem of the central
+    cross-section (D_yy + k0^2 eps) f = kappa^2 f and the discrete x
+    dispersion kappa^2 = (2 - 2 cos(beta dl)) / dl^2."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    dl, Ny = sim.dl, sim.Ny
+    xm = sim.Nx // 2
+    k0 = sim.omega / 299792458.0 * sim.L0
+    chi_int = sim.nonlinearity[0].chi
+    i = np.arange(Ny)
+    Dyf = sp.csr_matrix((np.r_[-np.ones(Ny), np.ones(Ny)] / dl, (np.r_[i, i], np.r_[i, (i + 1) % Ny])), shape=(Ny, Ny))
+    Dyb = sp.csr_matrix((np.r_[np.ones(Ny), -np.ones(Ny)] / dl, (np.r_[i, i], np.r_[i, (i - 1) % Ny])), shape=(Ny, Ny))
+    eps_col = np.asarray(sim.eps_r)[xm, :].astype(complex)
+    ph = np.unwrap(np.angle(E[xm - 50:xm + 50, Ny // 2]))
+    beta_guess = abs(np.polyfit(np.arange(100) * dl, ph, 1)[0])
+
+    def kappa2(e):
+        vals, vecs = spl.eigs((sp.diags(k0 ** 2 * e) + Dyf @ Dyb).tocsc(), k=1, sigma=beta_guess ** 2, which='LM')
+        return np.real(vals[0])
+    deps = 3 * chi_int * region[xm, :] * np.abs(E[xm, :]) ** 2
+    k2a, k2b = kappa2(eps_col), kappa2(eps_col + deps)
+    beta = lambda k2: np.arccos(1 - k2 * dl ** 2 / 2) / dl
+    return (beta(k2b) - beta(k2a)) * L / np.pi
+
+
 def test_chi3_self_phase_modulation_matches_analytic():
     sim, g = _kerr_waveguide(eps_scale=False)
-    Nx, Ny, n0, dl, chi3 = g['Nx'], g['Ny'], g['n0'], g['dl'], g['chi3']
+    Nx, Ny, dl, chi3 = g['Nx'], g['Ny'], g['dl'], g['chi3']
+    region = np.zeros((Nx, Ny))
+    region[int(Nx / 2 - g['lv'] / 2):int(Nx / 2 + g['lv'] / 2), int(Ny / 2 - g['wv'] / 2):int(Ny / 2 + g['wv'] / 2)] = 1
     sim.solve_fields()
-    T_linear = sim.fields['Ez'][Nx - 20, int(Ny / 2)] / sim.fields['Ez'][20, int(Ny / 2)]
-    T_vec, pwr_vec = [], []
+    E_lin = np.array(sim.fields['Ez'])
+    src1 = np.array(sim.src)
+    yc = int(Ny / 2)
+    T_linear = E_lin[Nx - 20, yc] / E_lin[20, yc]
+    phi_mode1 = _mode_phase_per_amp2(sim, E_lin, region, g['lv'] * dl)
+    Phi_kerr, Phi_pert, Phi_mode = [], [], []
     for srcval in np.logspace(1, 3, 3):
         sim.setup_modes()
         sim.src = sim.src * srcval
         sim.fields = {k: None for k in sim.fields}    # Newton starts from this source's linear field
         sim.solve_fields_nl(solver_nl='newton', Estart=None)
         E = np.array(sim.fields_nl['Ez'])
-        T_vec.append(E[Nx - 20, int(Ny / 2)] / E[20, int(Ny / 2)])
-        pwr_vec.append(sim.flux_probe('x', [Nx - 20, int(Ny / 2)], g['wv'] * 3, nl=True))
-    T_vec, pwr_vec = np.array(T_vec), np.array(pwr_vec)
-    # test_chi3.py:63-75: n2 from chi3, gamma = k0 n2 / Aeff, phase gamma P L
-    n2 = 12 * np.square(np.pi) * chi3 * 1e4 / np.square(n0)
-    n2 *= 1e-4 / np.square(sim.L0)
-    width = dl * g['wv']
-    height = width
-    Aeff = width * height
-    L = dl * g['lv']
-    gamma_spm = (sim.omega / 299792458 * sim.L0) * n2 / Aeff
-    Phi_fdfd = -np.unwrap(np.angle(T_vec) - np.angle(T_linear)) / np.pi
-    Phi_analytic = (pwr_vec * height) * L * gamma_spm / np.pi
-    print("Phi_fdfd", Phi_fdfd, "Phi_analytic", Phi_analytic, "ratio", Phi_fdfd / Phi_analytic)
-    np.testing.assert_allclose(Phi_fdfd, Phi_analytic, rtol=1e-3)
+        Phi_kerr.append(-np.unwrap([np.angle(E[Nx - 20, yc] / E[20, yc]) - np.angle(T_linear)])[0] / np.pi)
+        # linear solve with the permittivity the Kerr term gives the linear field
+        deps = 3 * chi3 / np.square(sim.L0) * region * np.abs(srcval * E_lin) ** 2
+        lin = Simulation(sim.omega, np.asarray(sim.eps_r) + deps, dl, [15, 15], 'Ez')
+        lin.src = src1 * srcval
+        _, _, Ep = lin.solve_fields()
+        Phi_pert.append(-np.unwrap([np.angle(Ep[Nx - 20, yc] / Ep[20, yc]) - np.angle(T_linear)])[0] / np.pi)
+        Phi_mode.append(phi_mode1 * srcval ** 2)
+    Phi_kerr, Phi_pert, Phi_mode = map(np.array, (Phi_kerr, Phi_pert, Phi_mode))
+    print("Phi Kerr", Phi_kerr, "first order (linear solve)", Phi_pert, "guided-mode theory", Phi_mode,
+          "ratios", Phi_kerr / Phi_pert, Phi_kerr / Phi_mode)
+    assert np.all(Phi_kerr > 0)                           # the Kerr term raises the index
+    np.testing.assert_allclose(Phi_kerr[:2], Phi_pert[:2], rtol=1e-3)
+    np.testing.assert_allclose(Phi_kerr[2], Phi_pert[2], rtol=1e-2)   # second order shows at 1e3
+    np.testing.assert_allclose(Phi_kerr[1] / Phi_kerr[0], 100, rtol=1e-3)   # SPM: phase ~ power
+    np.testing.assert_allclose(Phi_kerr, Phi_mode, rtol=0.06)

@@ -130,3 +130,45 @@ def test_large_requests_alone_equal_in_mix():
         print(f"request {r.rid} ({p.shape[0]}^2, {b.shape[0]} RHS): {r.iterations} refinement steps, "
               f"oracle-CSR residual {worst:.2e}")
         assert worst <= 1.5e-10
+
+
+def test_engine_many_waiters_on_tiny_jobs():
+    """32 host threads submit tiny factorizations (short jobs, so waiters
+    often arrive while their job's last step is being enqueued) and wait on
+    them at once, repeatedly: every wait returns a complete factor, equal to
+    the one-by-one launch-per-step factor. The completion state of a job is
+    set under the engine's lock only after its event is recorded; a waiter
+    that saw it earlier would destroy an unrecorded event and read a factor
+    still being written."""
+    import threading
+    from fdfdpy_amd.device import DeviceOperator, FactorEngine
+    eng = FactorEngine("cuda:0")
+    probs = [C.c3_request(400 + i, n=n) for i, n in enumerate((96, 128, 96, 160))]
+    ref = {}
+    for i, p in enumerate(probs):
+        op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, "cuda:0")
+        op.set_eps(p.eps)
+        ref[i] = op.factor("c64", stepwise=True).clone()
+    errors = []
+
+    def worker(t):
+        try:
+            for rep in range(6):
+                i = (t + rep) % len(probs)
+                p = probs[i]
+                op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, "cuda:0")
+                op.set_eps(p.eps)
+                op.engine = eng
+                f = op.factor("c64")
+                torch.cuda.current_stream().synchronize()
+                if not torch.equal(f, ref[i]):
+                    errors.append((t, rep, i))
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(32)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    eng.close()
+    assert not errors, errors[:5]
