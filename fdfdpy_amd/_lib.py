@@ -72,6 +72,8 @@ SIGNATURES = {
     "fdfd_sweep": (c_int, [_G, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "fdfd_solve": (c_int, [_G, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_double,
                            c_int, c_void_p, POINTER(FdfdSolveStats), POINTER(c_double), c_void_p]),
+    "fdfd_solve_static": (c_int, [_G, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_double,
+                                  c_int, POINTER(FdfdSolveStats), POINTER(c_double), c_void_p]),
     "fdfd_norms": (c_int, [_G, c_void_p, c_int, c_void_p, POINTER(c_double), c_void_p]),
     "fdfd_stretch_scale": (c_int, [_G, c_void_p, c_int, c_int, c_void_p]),
     "fdfd_grad_eps": (c_int, [_G, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -3150,6 +3150,33 @@ int solve_impl(const fdfd_grid *g, const double *op, const void *factor, int pre
   return FDFD_OK;
 }
 
+// ------------------------------------------- solve without allocations
+// The workspace of fdfd_solve_static: one array per device, at a fixed
+// address for the life of the process. Host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device (the array's
+// name is a host-side stand-in, not a device address).
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int STATIC_MAX_RHS = 16;
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+// this device's copy of scratch (looked up once per device)
+int static_scratch(void **ptr) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(FDFD_ERR_ARG, "device %d beyond the supported %d", dev, MAX_DEVICES);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(FDFD_ERR_HIP, "hipGetSymbolAddress returned NULL for the solve scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return FDFD_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -3164,6 +3191,27 @@ int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int pre
   std::vector<double> host(4 * (size_t)nrhs);
   return solve_impl(g, op, factor, precision, b, x, nrhs, transpose, tol, max_iter, work, stats, rel_res,
                     stream, host.data());
+}
+
+int fdfd_solve_static(const fdfd_grid *g, const double *op, const void *factor, int precision,
+                      const double *b, double *x, int nrhs, int transpose, double tol, int max_iter,
+                      fdfd_solve_stats *stats, double *rel_res, hipStream_t stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (int rc = check_prec(precision)) return rc;
+  if (!op || !factor || !b || !x || nrhs < 1 || max_iter < 1 || !(tol > 0))
+    return fail(FDFD_ERR_ARG, "bad arguments to fdfd_solve_static");
+  if (nrhs > STATIC_MAX_RHS)
+    return fail(FDFD_ERR_ARG, "fdfd_solve_static: %d right-hand sides exceed the supported %d", nrhs,
+                STATIC_MAX_RHS);
+  const size_t need = solve_work_bytes(g, precision, nrhs);
+  if (need > STATIC_SCRATCH_BYTES)
+    return fail(FDFD_ERR_ARG, "fdfd_solve_static: the solve needs %zu bytes of workspace, beyond the static "
+                "%zu (use fdfd_solve with a caller workspace)", need, STATIC_SCRATCH_BYTES);
+  void *work = nullptr;
+  if (int rc = static_scratch(&work)) return rc;
+  double host[4 * STATIC_MAX_RHS];
+  return solve_impl(g, op, factor, precision, b, x, nrhs, transpose, tol, max_iter, work, stats, rel_res,
+                    stream, host);
 }
 
 int fdfd_grad_eps(const fdfd_grid *g, const double *lam, const double *e, double *grad,

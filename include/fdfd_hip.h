@@ -178,6 +178,21 @@ int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int pre
                int max_iter, void *work, fdfd_solve_stats *stats, double *rel_res,
                hipStream_t stream);
 
+/* fdfd_solve with no workspace argument and no allocation: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * The workspace is a static 64 MiB array of the library on each device
+ * (held until the process ends); a solve whose fdfd_solve_workspace_bytes
+ * exceeds it, or with more than 16 right-hand sides, fails with FDFD_ERR_ARG
+ * (use fdfd_solve). Every call on one device shares that array: calls that
+ * overlap on one GPU (other streams, other host threads) race and must be
+ * serialized by the caller. Nothing in the array needs to start at zero; the
+ * solve clears what it reads before reading it. Other arguments, results
+ * and waiting as for fdfd_solve. */
+int fdfd_solve_static(const fdfd_grid *g, const double *op, const void *factor, int precision,
+                      const double *b, double *x, int nrhs, int transpose, double tol,
+                      int max_iter, fdfd_solve_stats *stats, double *rel_res, hipStream_t stream);
+
 /* Per-RHS squared 2-norms of v (device) into host out[nrhs]; waits. */
 int fdfd_norms(const fdfd_grid *g, const double *v, int nrhs, void *work,
                double *out, hipStream_t stream);
