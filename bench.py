@@ -16,8 +16,10 @@ value) and for the FP64-factor path on the same trace.
 Two loads: the throughput leg offers the stream faster than it is served
 (Poisson arrivals at --rate), so solves/s is the saturated rate and its
 latencies include queueing; the latency leg replays the first
---latency-requests requests at 0.5 x the FP64 path's measured request rate,
-below both paths' capacity, and its p50/p99 are the headline latencies.
+--latency-requests requests at 0.3 x the FP64 path's saturated request rate,
+below both paths' capacity at that load (the factorizations batch less when
+fewer requests are in flight, so capacity falls with the load), and its
+p50/p99 are the headline latencies.
 
 Also reported: the roofline of the dominant kernel (the Gauss-Jordan step
 of the complex64 factor, 8 factorizations per launch as in the stream,
@@ -67,9 +69,9 @@ def parse():
                     help="wall-time budget of the CPU pool's sample of the stream")
     ap.add_argument("--cpu-workers", type=int, default=0,
                     help="CPU pool size (0: the host cores of this process, at most 16)")
-    ap.add_argument("--latency-requests", type=int, default=256,
+    ap.add_argument("--latency-requests", type=int, default=160,
                     help="requests of the latency leg (arrivals below capacity)")
-    ap.add_argument("--latency-load", type=float, default=0.5,
+    ap.add_argument("--latency-load", type=float, default=0.3,
                     help="arrival rate of the latency leg as a fraction of the FP64 path's request rate")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 byte-counter passes")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
@@ -250,7 +252,7 @@ def kernels_only(n=ROOF_N, njobs=ROOF_JOBS, nrhs=4):
         out[prec + "_batched_factor_ms"] = round(_batched_factor(ops, prec, stream_timed=False), 3)
         b = torch.randn((nrhs,) + ops[0].shape, dtype=torch.complex128, generator=rng).to("cuda:0")
         x, rep = ops[0].solve(b, precision=prec, tol=1e-10)
-        out[prec + "_solve"] = {"steps": rep.iterations, "residual": rep.max_rel_residual}
+        out[prec + "_solve"] = {"steps": rep.iterations, "sweeps": rep.sweeps, "residual": rep.max_rel_residual}
         for op in ops:
             op.drop_factor()
     torch.cuda.synchronize()
@@ -434,7 +436,7 @@ def config_timings():
     xm, xf = res["c64"][0], res["fp64"][0]
     err = [float(torch.linalg.vector_norm(xm[q] - xf[q]) / torch.linalg.vector_norm(xf[q])) for q in range(8)]
     out["c2_design_Hz_512_8rhs"] = {
-        prec: {"factor_s": round(r[2], 4), "solve_8rhs_s": round(r[3], 4), "steps": r[1].iterations,
+        prec: {"factor_s": round(r[2], 4), "solve_8rhs_s": round(r[3], 4), "steps": r[1].iterations, "sweeps": r[1].sweeps,
                "max_rel_residual": r[1].max_rel_residual} for prec, r in res.items()}
     out["c2_design_Hz_512_8rhs"]["mixed_vs_fp64_max_rel_field_diff"] = max(err)
     # 4: Ez 300x300 adjoint gradient over the whole grid (forward + adjoint solve, resident factor)
@@ -507,7 +509,7 @@ def _path_summary(summ):
             "p50_service_ms": round(summ["p50_service_s"] * 1e3, 3),
             "p99_service_ms": round(summ["p99_service_s"] * 1e3, 3),
             "max_rel_residual": summ["max_rel_residual"], "failed": summ["failed"],
-            "fallbacks": summ["fallbacks"], "mean_sweeps": summ["mean_sweeps"]}
+            "fallbacks": summ["fallbacks"], "mean_steps": summ["mean_steps"], "mean_sweeps": summ["mean_sweeps"]}
 
 
 def main():
@@ -577,7 +579,7 @@ def main():
     if args.stream_only:
         print(json.dumps({"stream_c64": mixed, "stream_fp64": fp64}), flush=True)
         return
-    # latency leg: a prefix of the stream at 0.5 x the FP64 path's request rate
+    # latency leg: a prefix of the stream at 0.3 x the FP64 path's saturated request rate
     cap = (fp64 or mixed)["requests_per_s"]
     lat_rate = args.latency_load * cap
     k = min(args.latency_requests, n_req)
@@ -617,7 +619,7 @@ def main():
         "p99_service_ms": round(lat_m["p99_service_s"] * 1e3, 3),
         "max_rel_residual": max(mixed["max_rel_residual"], lat_m["max_rel_residual"]),
         "fallbacks": mixed["fallbacks"], "failed": mixed["failed"] + lat_m["failed"],
-        "mean_sweeps": mixed["mean_sweeps"],
+        "mean_steps": mixed["mean_steps"], "mean_sweeps": mixed["mean_sweeps"],
         "saturated": _path_summary(mixed) | {"engine_launches": mixed.get("engine_launches"),
                                              "jobs_per_launch": mixed.get("jobs_per_launch")},
         "roofline": {k_: roof[k_] for k_ in ("bound", "achieved", "peak", "unit", "frac", "traffic")},

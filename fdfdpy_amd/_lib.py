@@ -33,7 +33,7 @@ class FdfdGrid(ctypes.Structure):
 class FdfdSolveStats(ctypes.Structure):
     _fields_ = [("iterations", c_int32), ("converged", c_int32), ("stalled", c_int32),
                 ("zero_rhs", c_int32), ("max_rel_residual", c_double), ("restarts", c_int32),
-                ("reserved", c_int32)]
+                ("sweeps", c_int32)]
 
 
 class FdfdError(RuntimeError):

@@ -175,23 +175,33 @@ __global__ void __launch_bounds__(NT) k_assemble(fdfd_grid g, const double *__re
 }
 
 // ------------------------------------------------- stencil y = A x / b - A x
-__global__ void __launch_bounds__(NT) k_stencil(int nx, int ny, const double *__restrict__ op,
+// One thread per cell for all right-hand sides: the five coefficients are read
+// once and applied to every x[r]. b may alias y (each element is read by the
+// thread that writes it); x must not.
+__global__ void __launch_bounds__(NT) k_stencil(int nx, int ny, int nrhs, const double *__restrict__ op,
                                                 const double *__restrict__ x,
                                                 const double *b, double *y) {
   const int j = blockIdx.x * NT + threadIdx.x;
-  const int i = blockIdx.y, r = blockIdx.z;
+  const int i = blockIdx.y;
   if (j >= ny) return;
   const size_t M = (size_t)nx * ny, p = (size_t)i * ny + j;
-  const double *xr = x + 2 * M * r;
   const int ip = (i + 1) % nx, im = (i + nx - 1) % nx;
   const int jp = (j + 1) % ny, jm = (j + ny - 1) % ny;
-  cd acc = cmul(ld_c(op, FDFD_OP_C * M + p), ld_c(xr, p));
-  acc = cadd(acc, cmul(ld_c(op, FDFD_OP_XM * M + p), ld_c(xr, (size_t)im * ny + j)));
-  acc = cadd(acc, cmul(ld_c(op, FDFD_OP_XP * M + p), ld_c(xr, (size_t)ip * ny + j)));
-  acc = cadd(acc, cmul(ld_c(op, FDFD_OP_YM * M + p), ld_c(xr, (size_t)i * ny + jm)));
-  acc = cadd(acc, cmul(ld_c(op, FDFD_OP_YP * M + p), ld_c(xr, (size_t)i * ny + jp)));
-  if (b) acc = csub(ld_c(b + 2 * M * r, p), acc);
-  st_c(y + 2 * M * r, p, acc);
+  const size_t pxm = (size_t)im * ny + j, pxp = (size_t)ip * ny + j;
+  const size_t pym = (size_t)i * ny + jm, pyp = (size_t)i * ny + jp;
+  const cd c0 = ld_c(op, FDFD_OP_C * M + p), cxm = ld_c(op, FDFD_OP_XM * M + p);
+  const cd cxp = ld_c(op, FDFD_OP_XP * M + p), cym = ld_c(op, FDFD_OP_YM * M + p);
+  const cd cyp = ld_c(op, FDFD_OP_YP * M + p);
+  for (int r = 0; r < nrhs; ++r) {
+    const double *xr = x + 2 * M * r;
+    cd acc = cmul(c0, ld_c(xr, p));
+    acc = cadd(acc, cmul(cxm, ld_c(xr, pxm)));
+    acc = cadd(acc, cmul(cxp, ld_c(xr, pxp)));
+    acc = cadd(acc, cmul(cym, ld_c(xr, pym)));
+    acc = cadd(acc, cmul(cyp, ld_c(xr, pyp)));
+    if (b) acc = csub(ld_c(b + 2 * M * r, p), acc);
+    st_c(y + 2 * M * r, p, acc);
+  }
 }
 
 // ------------------------------------------------------------------ MFMA
@@ -2093,18 +2103,49 @@ __global__ void __launch_bounds__(NT) k_wrap_gemv(int n, const T *__restrict__ M
   }
 }
 
-// f = -U C w for the truncated wrap: block 0 gets -XM_0 w1, block m-1 gets
-// -XP_{m-1} w2 (f is zero elsewhere)
-__global__ void __launch_bounds__(NT) k_wrap_f(fdfd_grid g, const double *__restrict__ op,
-                                               const double *__restrict__ w1, const double *__restrict__ w2,
-                                               double *__restrict__ f) {
+// Lagged truncated wrap (one sweep per refinement step). Per right-hand side
+// q, five n-vectors sit in blocks 0..4 of the otherwise unused f buffer
+// (plane q of [nrhs][m][n], m > 5 on this path): g0, gl, the boundary part of
+// the residual left by the last step's coupling (blocks 0 and m-1); tl =
+// G_{m-1} gl; f0, fl, the wrap forcing added to blocks 0 and m-1 of r.
+enum { LAG_G0 = 0, LAG_GL = 1, LAG_TL = 2, LAG_F0 = 3, LAG_FL = 4 };
+
+// f0 = -XM_0 w1, fl = -XP_{m-1} w2 (kept for the next step's g) and added to
+// blocks 0 and m-1 of r
+__global__ void __launch_bounds__(NT) k_wrap_f_lag(fdfd_grid g, const double *__restrict__ op,
+                                                   const double *__restrict__ w1, const double *__restrict__ w2,
+                                                   double *__restrict__ lag, double *__restrict__ r) {
   const int n = g.ny, m = g.nx, i = blockIdx.x * NT + threadIdx.x, q = blockIdx.y;
   if (i >= n) return;
   const size_t M = (size_t)m * n;
   const cd a = cmul(ld_c(op, FDFD_OP_XM * M + i), ld_c(w1 + 2 * (size_t)n * q, i));
   const cd b = cmul(ld_c(op, FDFD_OP_XP * M + (size_t)(m - 1) * n + i), ld_c(w2 + 2 * (size_t)n * q, i));
-  st_c(f + 2 * M * q, i, cd{-a.r, -a.i});
-  st_c(f + 2 * M * q, (size_t)(m - 1) * n + i, cd{-b.r, -b.i});
+  double *lq = lag + 2 * M * q, *rq = r + 2 * M * q;
+  st_c(lq, (size_t)LAG_F0 * n + i, cd{-a.r, -a.i});
+  st_c(lq, (size_t)LAG_FL * n + i, cd{-b.r, -b.i});
+  st_c(rq, i, csub(ld_c(rq, i), a));
+  st_c(rq, (size_t)(m - 1) * n + i, csub(ld_c(rq, (size_t)(m - 1) * n + i), b));
+}
+
+// After the sweep s = T^-1 (r + f): the residual's boundary part is
+// g0 = -(f0 + XM_0 s_{m-1}) on block 0 and gl = -(fl + XP_{m-1} s_0) on block
+// m-1 (A s = T s + E s = r + f + E s). with_f = 0: no forcing was added.
+__global__ void __launch_bounds__(NT) k_wrap_lag_next(fdfd_grid g, const double *__restrict__ op,
+                                                      const double *__restrict__ s, int with_f,
+                                                      double *__restrict__ lag) {
+  const int n = g.ny, m = g.nx, i = blockIdx.x * NT + threadIdx.x, q = blockIdx.y;
+  if (i >= n) return;
+  const size_t M = (size_t)m * n;
+  const double *sq = s + 2 * M * q;
+  double *lq = lag + 2 * M * q;
+  cd g0 = cmul(ld_c(op, FDFD_OP_XM * M + i), ld_c(sq, (size_t)(m - 1) * n + i));
+  cd gl = cmul(ld_c(op, FDFD_OP_XP * M + (size_t)(m - 1) * n + i), ld_c(sq, i));
+  if (with_f) {
+    g0 = cadd(g0, ld_c(lq, (size_t)LAG_F0 * n + i));
+    gl = cadd(gl, ld_c(lq, (size_t)LAG_FL * n + i));
+  }
+  st_c(lq, (size_t)LAG_G0 * n + i, cd{-g0.r, -g0.i});
+  st_c(lq, (size_t)LAG_GL * n + i, cd{-gl.r, -gl.i});
 }
 
 // ------------------------------------------------------------------ norms
@@ -2279,7 +2320,7 @@ size_t factor_work_bytes(const fdfd_grid *g, int precision) {
 struct SolveWork {
   double *r; void *y; double *part; double *norms; SweepSync *sync;
   unsigned tag_next;     // tag base of the next persistent sweep launch (host side)
-  double *t, *f;         // wrap correction: T^-1 r and -U C K^-1 V^T t
+  double *t, *f;         // T^-1 r; the exact wrap's -U C K^-1 V^T t, or the lagged wrap's vectors
   double *u, *w1, *w2;   // truncated wrap: [nrhs][n] complex each
 };
 SolveWork solve_work_layout(const fdfd_grid *g, int precision, int nrhs, void *work) {
@@ -2573,31 +2614,42 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
   return FDFD_OK;
 }
 
-// f = -U C K^-1 V^T t for the truncated wrap (K^-1 through S^-1, Sigma_0 and
-// G_{m-1}): u = t_0 - Sigma_0 (XM_0 t_{m-1}), w2 = S^-1 u,
-// w1 = t_{m-1} - G_{m-1} (XP_{m-1} w2), then f from w1, w2.
+// The truncated wrap correction, lagged: the forcing f that makes
+// T^-1 (r + f) correct the boundary part g of r. For t = T^-1 g,
+// f = -U C K^-1 V^T t through S^-1: u = t_0 - Sigma_0 (XM_0 t_{m-1}),
+// w2 = S^-1 u, w1 = t_{m-1} - G_{m-1} (XP_{m-1} w2), f = (-XM_0 w1 on block 0,
+// -XP_{m-1} w2 on block m-1), with t's boundary blocks taken as Sigma_0 g0
+// and G_{m-1} gl (the far corners dropped); f goes into r.
 template <typename T>
-int wrap_apply_truncated(const fdfd_grid *g, const double *op, const void *factor, const SolveWork &sw, int nrhs,
-                         hipStream_t s) {
+int wrap_lag_forcing(const fdfd_grid *g, const double *op, const void *factor, const SolveWork &sw, int nrhs,
+                     double *r, hipStream_t s) {
   const int prec = sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64;
   const int m = g->nx, n = g->ny, np = pad_n(n);
-  const size_t M = (size_t)m * n, np2 = (size_t)np * np;
+  const size_t M = (size_t)m * n, np2 = (size_t)np * np, nv = 2 * (size_t)n;
   const T *sigma = (const T *)wrap_sigma(g, prec, factor);
   const T *Glast = (const T *)factor + (size_t)(m - 1) * 2 * np2;
   const double *sinv = wrap_kinv(g, prec, factor);
   const double *XM0 = op + 2 * (FDFD_OP_XM * M), *XPl = op + 2 * (FDFD_OP_XP * M + (size_t)(m - 1) * n);
-  const double *t_last = sw.t + 2 * (size_t)(m - 1) * n;
+  double *lag = sw.f;
+  const double *g0 = lag + LAG_G0 * nv, *gl = lag + LAG_GL * nv;
+  double *tl = lag + LAG_TL * nv;
   const dim3 gg((n + NT / 64 - 1) / (NT / 64), nrhs);
-  hipLaunchKernelGGL(k_wrap_gemv<T>, gg, dim3(NT), 0, s, n, sigma, np, XM0, t_last, 2 * M,
-                     (const double *)sw.t, 2 * M, -1.0, sw.u, 2 * (size_t)n);
+  // tl = G_{m-1} gl
+  hipLaunchKernelGGL(k_wrap_gemv<T>, gg, dim3(NT), 0, s, n, Glast, np, (const double *)nullptr, gl, 2 * M,
+                     (const double *)nullptr, (size_t)0, 1.0, tl, 2 * M);
+  // u = Sigma_0 g0 - Sigma_0 (XM_0 tl)   (w1 holds Sigma_0 g0)
+  hipLaunchKernelGGL(k_wrap_gemv<T>, gg, dim3(NT), 0, s, n, sigma, np, (const double *)nullptr, g0, 2 * M,
+                     (const double *)nullptr, (size_t)0, 1.0, sw.w1, nv);
+  hipLaunchKernelGGL(k_wrap_gemv<T>, gg, dim3(NT), 0, s, n, sigma, np, XM0, (const double *)tl, 2 * M,
+                     (const double *)sw.w1, nv, -1.0, sw.u, nv);
+  // w2 = S^-1 u; w1 = tl - G_{m-1} (XP_{m-1} w2)
   hipLaunchKernelGGL(k_wrap_gemv<double>, gg, dim3(NT), 0, s, n, sinv, wrap_sinv_tiled(g, prec) ? np : 0,
-                     (const double *)nullptr,
-                     (const double *)sw.u, 2 * (size_t)n, (const double *)nullptr, (size_t)0, 1.0, sw.w2,
-                     2 * (size_t)n);
-  hipLaunchKernelGGL(k_wrap_gemv<T>, gg, dim3(NT), 0, s, n, Glast, np, XPl, (const double *)sw.w2,
-                     2 * (size_t)n, t_last, 2 * M, -1.0, sw.w1, 2 * (size_t)n);
-  hipLaunchKernelGGL(k_wrap_f, dim3((n + NT - 1) / NT, nrhs), dim3(NT), 0, s, *g, op, (const double *)sw.w1,
-                     (const double *)sw.w2, sw.f);
+                     (const double *)nullptr, (const double *)sw.u, nv, (const double *)nullptr, (size_t)0, 1.0,
+                     sw.w2, nv);
+  hipLaunchKernelGGL(k_wrap_gemv<T>, gg, dim3(NT), 0, s, n, Glast, np, XPl, (const double *)sw.w2, nv,
+                     (const double *)tl, 2 * M, -1.0, sw.w1, nv);
+  hipLaunchKernelGGL(k_wrap_f_lag, dim3((n + NT - 1) / NT, nrhs), dim3(NT), 0, s, *g, op, (const double *)sw.w1,
+                     (const double *)sw.w2, lag, r);
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
 }
@@ -2915,8 +2967,8 @@ int fdfd_apply(const fdfd_grid *g, const double *op, const double *x, double *y,
                hipStream_t stream) {
   if (int rc = check_grid(g)) return rc;
   if (!op || !x || !y || nrhs < 1) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_apply");
-  hipLaunchKernelGGL(k_stencil, dim3((g->ny + NT - 1) / NT, g->nx, nrhs), dim3(NT), 0, stream,
-                     g->nx, g->ny, op, x, (const double *)nullptr, y);
+  hipLaunchKernelGGL(k_stencil, dim3((g->ny + NT - 1) / NT, g->nx), dim3(NT), 0, stream,
+                     g->nx, g->ny, nrhs, op, x, (const double *)nullptr, y);
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
 }
@@ -2925,8 +2977,8 @@ int fdfd_residual(const fdfd_grid *g, const double *op, const double *b, const d
                   double *r, int nrhs, hipStream_t stream) {
   if (int rc = check_grid(g)) return rc;
   if (!op || !x || !b || !r || nrhs < 1) return fail(FDFD_ERR_ARG, "bad arguments to fdfd_residual");
-  hipLaunchKernelGGL(k_stencil, dim3((g->ny + NT - 1) / NT, g->nx, nrhs), dim3(NT), 0, stream,
-                     g->nx, g->ny, op, x, b, r);
+  hipLaunchKernelGGL(k_stencil, dim3((g->ny + NT - 1) / NT, g->nx), dim3(NT), 0, stream,
+                     g->nx, g->ny, nrhs, op, x, b, r);
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
 }
@@ -3049,7 +3101,7 @@ int solve_impl(const fdfd_grid *g, const double *op, const void *factor, int pre
                fdfd_solve_stats *stats, double *rel_res, hipStream_t stream, double *host) {
   SolveWork sw = solve_work_layout(g, precision, nrhs, work);
   const size_t M = (size_t)g->nx * g->ny;
-  const dim3 sgrid((g->ny + NT - 1) / NT, g->nx, nrhs);
+  const dim3 sgrid((g->ny + NT - 1) / NT, g->nx);
   fdfd_solve_stats st;
   memset(&st, 0, sizeof st);
   double *nb = host, *nr = host + nrhs, *rel = host + 2 * nrhs, *prev = host + 3 * nrhs;
@@ -3086,30 +3138,44 @@ int solve_impl(const fdfd_grid *g, const double *op, const void *factor, int pre
     const double *kinv = wrap_kinv(g, precision, factor);
     const bool exact = wrap_exact(g);
     for (int it = 1; it <= max_iter; ++it) {
-      // t = T^-1 r; f = -U C K^-1 V^T t; t += T^-1 f; x += t  (x += A^-1 r)
       HIP_TRY(hipMemsetAsync(sw.t, 0, 16 * M * nrhs, stream));
-      rc = fp64 ? sweep_impl<double>(g, op, factor, rr, sw.t, nrhs, sw, stream, blockwise)
-                : sweep_impl<float>(g, op, factor, rr, sw.t, nrhs, sw, stream, blockwise);
-      if (rc) return rc;
-      HIP_TRY(hipMemsetAsync(sw.f, 0, 16 * M * nrhs, stream));
       if (exact) {
+        // t = T^-1 r; f = -U C K^-1 V^T t; t += T^-1 f; x += t  (x += A^-1 r)
+        rc = fp64 ? sweep_impl<double>(g, op, factor, rr, sw.t, nrhs, sw, stream, blockwise)
+                  : sweep_impl<float>(g, op, factor, rr, sw.t, nrhs, sw, stream, blockwise);
+        if (rc) return rc;
+        HIP_TRY(hipMemsetAsync(sw.f, 0, 16 * M * nrhs, stream));
         hipLaunchKernelGGL(k_wrap_rhs, dim3((2 * g->ny + NT / 64 - 1) / (NT / 64), nrhs), dim3(NT), 0, stream, *g,
                            op, kinv, (const double *)sw.t, sw.f);
         HIP_TRY(hipGetLastError());
-      } else if ((rc = fp64 ? wrap_apply_truncated<double>(g, op, factor, sw, nrhs, stream)
-                            : wrap_apply_truncated<float>(g, op, factor, sw, nrhs, stream))) {
-        return rc;
+        rc = fp64 ? sweep_impl<double>(g, op, factor, sw.f, sw.t, nrhs, sw, stream, blockwise)
+                  : sweep_impl<float>(g, op, factor, sw.f, sw.t, nrhs, sw, stream, blockwise);
+        if (rc) return rc;
+        st.sweeps += 2;
+      } else {
+        // One sweep per step. The previous step left r = g + d, g on the
+        // boundary blocks (its uncorrected wrap coupling) and d small. The
+        // forcing f corrects g exactly (truncation aside): x += T^-1 (r + f),
+        // and the new boundary part g' follows from f and the sweep's
+        // boundary blocks. The first step has no g.
+        if (it > 1 && (rc = fp64 ? wrap_lag_forcing<double>(g, op, factor, sw, nrhs, rr, stream)
+                                 : wrap_lag_forcing<float>(g, op, factor, sw, nrhs, rr, stream)))
+          return rc;
+        rc = fp64 ? sweep_impl<double>(g, op, factor, rr, sw.t, nrhs, sw, stream, blockwise)
+                  : sweep_impl<float>(g, op, factor, rr, sw.t, nrhs, sw, stream, blockwise);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_wrap_lag_next, dim3((g->ny + NT - 1) / NT, nrhs), dim3(NT), 0, stream, *g, op,
+                           (const double *)sw.t, it > 1 ? 1 : 0, sw.f);
+        HIP_TRY(hipGetLastError());
+        st.sweeps += 1;
       }
-      rc = fp64 ? sweep_impl<double>(g, op, factor, sw.f, sw.t, nrhs, sw, stream, blockwise)
-                : sweep_impl<float>(g, op, factor, sw.f, sw.t, nrhs, sw, stream, blockwise);
-      if (rc) return rc;
       hipLaunchKernelGGL(k_axpy, dim3(1024), dim3(NT), 0, stream, 2 * M * nrhs, (const double *)sw.t, x);
       HIP_TRY(hipGetLastError());
       st.iterations = it;
       // r = b' - A x, with b' = b or D^-1 b
       HIP_TRY(hipMemcpyAsync(rr, b, 16 * M * nrhs, hipMemcpyDeviceToDevice, stream));
       if (transpose && (rc = fdfd_stretch_scale(g, rr, nrhs, 1, stream))) return rc;
-      hipLaunchKernelGGL(k_stencil, sgrid, dim3(NT), 0, stream, g->nx, g->ny, op, (const double *)x,
+      hipLaunchKernelGGL(k_stencil, sgrid, dim3(NT), 0, stream, g->nx, g->ny, nrhs, op, (const double *)x,
                          (const double *)rr, rr);
       HIP_TRY(hipGetLastError());
       // transposed: ||b - A^T x|| = ||D (b' - A x')||

@@ -74,6 +74,7 @@ class SolveReport:
     fell_back: bool = False     # solved with a factor other than the requested one
     pivoted: bool = False       # the factor pivots inside its 32x32 pivot tiles
     restarts: int = 0           # persistent sweeps timed out; redone per block
+    sweeps: int = 0             # block sweeps T^-1 r of the solve that produced x
     failed_factors: list = field(default_factory=list)   # (precision, pivoted) with a zero/non-finite pivot
     factor_seconds: float = 0.0
     solve_seconds: float = 0.0
@@ -284,7 +285,7 @@ class DeviceOperator:
         if not rep.converged:
             raise L.NotConverged(L.FDFD_ERR_NOT_CONVERGED,
                                  f"solve did not reach rel. residual {tol:g}: {rep.max_rel_residual:.3e} "
-                                 f"after {rep.iterations} sweeps ({rep.precision})")
+                                 f"after {rep.iterations} refinement steps ({rep.precision})")
         out = self.to_user(x)
         return (out[0] if squeeze else out), rep
 
@@ -306,7 +307,7 @@ class DeviceOperator:
             L.check(rc)
         return SolveReport(precision=precision, iterations=st.iterations, converged=bool(st.converged),
                            max_rel_residual=st.max_rel_residual, rel_residual=list(rel), restarts=st.restarts,
-                           factor_seconds=tf, solve_seconds=time.perf_counter() - t1)
+                           sweeps=st.sweeps, factor_seconds=tf, solve_seconds=time.perf_counter() - t1)
 
     # -- stencil -------------------------------------------------------------
     def apply(self, x):

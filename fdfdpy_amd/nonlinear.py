@@ -219,7 +219,7 @@ def newton_solve(sim, Estart=None, conv_threshold=1e-10, max_num_iter=50, averag
             j12 = (w * coeff).to(torch.complex128) * Eprev * Eprev
             op.kerr_update(Eprev, 2 * coeff)
             # one factorization of J11 per step (the reference refactors its
-            # Jacobian every step too); the inner solves then refine in ~2 sweeps
+            # Jacobian every step too); the inner solves then take a few refinement steps
             op.factor(precision, refresh=True)
             E = Eprev - _conj_coupled_solve(op, j12, f, precision, NEWTON_INNER_TOL)
             nE = torch.linalg.vector_norm(E).item()

@@ -47,6 +47,7 @@ class Result:
     factor_reused: bool
     error: str = ""
     sweep_restarts: int = 0    # solves redone with per-block sweep launches
+    sweeps: int = 0            # block sweeps T^-1 r of the solve
 
     @property
     def latency(self):
@@ -199,7 +200,7 @@ class ServingEngine:
                          x if self.keep_fields else None, req.arrival, admitted, done,
                          rep.precision if rep else self.precision, rep.iterations if rep else 0,
                          rep.max_rel_residual if rep else float("nan"), rep.fell_back if rep else False,
-                         reused, err, rep.restarts if rep else 0)
+                         reused, err, rep.restarts if rep else 0, rep.sweeps if rep else 0)
             with cond:
                 results[req.rid] = res
                 state["used"] -= nbytes
@@ -252,6 +253,7 @@ def summarize(results, wall):
         "p50_service_s": float(np.percentile(svc, 50)), "p99_service_s": float(np.percentile(svc, 99)),
         "max_rel_residual": float(max((r.max_rel_residual for r in ok), default=float("nan"))),
         "fallbacks": sum(r.fell_back for r in ok),
-        "mean_sweeps": float(np.mean([r.iterations for r in ok])) if ok else float("nan"),
+        "mean_steps": float(np.mean([r.iterations for r in ok])) if ok else float("nan"),
+        "mean_sweeps": float(np.mean([r.sweeps for r in ok])) if ok else float("nan"),
         "sweep_restarts": sum(r.sweep_restarts for r in ok),
     }

@@ -80,7 +80,8 @@ typedef struct fdfd_solve_stats {
   double max_rel_residual;/* max over RHS of ||b - A x|| / ||b||            */
   int32_t restarts;       /* 1 when the persistent sweeps timed out and the  */
                           /* solve was redone with per-block launches       */
-  int32_t reserved;
+  int32_t sweeps;         /* block sweeps x += T^-1 r applied (forward and  */
+                          /* backward pass each) in the final attempt       */
 } fdfd_solve_stats;
 
 int fdfd_abi_version(void);
@@ -169,8 +170,12 @@ int fdfd_sweep(const fdfd_grid *g, const double *op, const void *factor, int pre
 /* Replaces pardisoSolver.solve(b) (linalg.py:137) and spsolve (linalg.py:141):
  * solves A x = b (transpose = 0) or A^T x = b (transpose = 1, the adjoint
  * system) to ||b - A x|| <= tol ||b|| per RHS by refinement with the
- * wrap-corrected factor (each iteration: two block sweeps and the
- * capacitance product), at most max_iter iterations. An all-zero b gives x = 0 without a sweep
+ * wrap-corrected factor, at most max_iter iterations. With the truncated
+ * wrap correction (PML of 4 or more cells on both axes) an iteration is one
+ * block sweep: the correction of the coupling the previous iteration left
+ * on the boundary blocks is added to its right-hand side. With the exact
+ * correction an iteration is two sweeps (T^-1 r, then T^-1 of the
+ * capacitance product's forcing). An all-zero b gives x = 0 without a sweep
  * (linalg.py:128-129). rel_res (optional, device-independent host array of
  * nrhs doubles) receives the final relative residuals. */
 int fdfd_solve(const fdfd_grid *g, const double *op, const void *factor, int precision,
