@@ -62,7 +62,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1, help="untimed batches (of 8 requests each)")
     ap.add_argument("--batch", type=int, default=64, help="requests per step")
     ap.add_argument("--rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
-    ap.add_argument("--slots", type=int, default=32, help="requests in flight")
+    ap.add_argument("--slots", type=int, default=48, help="requests in flight")
     ap.add_argument("--engines", type=int, default=2, help="factorization engines (streams of step launches)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--cpu-seconds", type=float, default=25.0,

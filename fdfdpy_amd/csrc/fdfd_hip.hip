@@ -430,14 +430,40 @@ __device__ inline void x_elem(const FJob<T> &J, int k, int p, const T *src, int 
   im = (T)vi;
 }
 
-// Six tiles of LDS per workgroup (34 KB -> 25 KB for complex64, 5 groups per
-// CU instead of 4): R = P X[p,J] replaces X[p,J] once every wave has read it,
-// and the tile inverse uses the P and X[I,p] buffers as its double buffer.
+// Exchange buffers of the tile inverses: rows padded to a 16-byte multiple so
+// that a thread's four consecutive elements move as one 16-byte LDS access
+// (complex64: 36 floats; FP64: 34 doubles). Two of them overlay the start of
+// the workgroup's LDS block while the tile itself is held in registers.
+template <typename T> constexpr int inv_stride() { return sizeof(T) == 8 ? 34 : 36; }
+template <typename T> struct InvX {
+  T r[TB][inv_stride<T>()];
+  T i[TB][inv_stride<T>()];
+};
+
+// LDS of the persistent factorization: three complex tiles. R = P X[p,J]
+// replaces X[p,J] once every wave has read it.
 template <typename T> struct Lds {
-  T Pr[TB][TBP], Pi[TB][TBP];  // pivot inverse; inversion buffer 0
+  T Pr[TB][TBP], Pi[TB][TBP];  // pivot inverse
   T Ar[TB][TBP], Ai[TB][TBP];  // X[p,J], then R = P X[p,J]; tile to invert / inverse
-  T Br[TB][TBP], Bi[TB][TBP];  // X[I,p]; inversion buffer 1
+  T Br[TB][TBP], Bi[TB][TBP];  // X[I,p]
   T mag[2][TB];                // pivot candidates (double-buffered)
+  int perm[TB];
+  int colmap[TB];
+};
+
+// LDS of the step kernel: two complex tiles (18.5 KB for complex64, eight
+// workgroups per CU). P shares the B buffer with X[I,p], which waits in
+// registers until R = P X[p,J] is formed; the pad makes room for the tile
+// inverse's two exchange buffers.
+template <typename T> constexpr int step_lds_pad() {
+  return 2 * (int)sizeof(InvX<T>) > 4 * TB * TBP * (int)sizeof(T)
+             ? 2 * (int)sizeof(InvX<T>) - 4 * TB * TBP * (int)sizeof(T) : 16;
+}
+template <typename T> struct LdsStep {
+  T Ar[TB][TBP], Ai[TB][TBP];  // X[p,J], then R; or P (column p); tile to invert / inverse
+  T Br[TB][TBP], Bi[TB][TBP];  // P, then X[I,p]
+  unsigned char pad[step_lds_pad<T>()];
+  T mag[2][TB];
   int perm[TB];
   int colmap[TB];
 };
@@ -490,15 +516,6 @@ __device__ inline void fast_crecip(T zr, T zi, T &outr, T &outi) {
   outi = -ai * rd;
 }
 
-// Exchange buffers of the tile inverse: rows padded to a 16-byte multiple so
-// that a thread's four consecutive elements move as one 16-byte LDS access
-// (complex64: 36 floats; FP64: 34 doubles). Two of them overlay the whole
-// Lds block while the tile itself is held in registers.
-template <typename T> constexpr int inv_stride() { return sizeof(T) == 8 ? 34 : 36; }
-template <typename T> struct InvX {
-  T r[TB][inv_stride<T>()];
-  T i[TB][inv_stride<T>()];
-};
 __device__ inline void lds_st4(float *p, const float (&v)[4]) {
   typedef float f4 __attribute__((ext_vector_type(4)));
   *reinterpret_cast<f4 *>(p) = f4{v[0], v[1], v[2], v[3]};
@@ -539,10 +556,10 @@ __device__ inline double exp_scale(double x) { return ldexp(1.0, -__builtin_amdg
 // columns with ten LDS accesses, and the pivot rows are handled by the same
 // update formula as the other rows (no divergent branch). Scales are exact
 // powers of two.
-template <typename T>
-__device__ void tile_inverse_diag(Lds<T> &L, int *status) {
+template <typename T, typename LT>
+__device__ void tile_inverse_diag(LT &L, int *status) {
   static_assert(TB * TB == 4 * NT, "4 elements per thread");
-  static_assert(2 * sizeof(InvX<T>) <= sizeof(Lds<T>), "exchange buffers overlay the Lds block");
+  static_assert(2 * sizeof(InvX<T>) <= offsetof(LT, mag), "exchange buffers overlay the tiles only");
   const int tid = threadIdx.x;
   const int i = tid >> 3, c0 = (tid & 7) * 4;
   T ar[4], ai[4];
@@ -624,24 +641,27 @@ __device__ void tile_inverse_diag(Lds<T> &L, int *status) {
 }
 
 // Inverse of the TB x TB tile in (Ar, Ai), in place, by Gauss-Jordan with
-// partial pivoting inside the tile (the fallback factorization). Thread t keeps row t/8, columns
-// 4(t%8)..+3 in registers; each pivot step publishes the tile to one of two
-// LDS buffers (Pr/Pi, Br/Bi, alternating), so one barrier per pivot orders
-// the exchange. The row interchanges become one permuted write-back.
-template <typename T>
-__device__ void tile_inverse_pivoted(Lds<T> &L, int *status) {
+// partial pivoting inside the tile (the fallback factorization). Thread t
+// keeps row t/8, columns 4(t%8)..+3 in registers; each pivot step publishes
+// the tile to one of the two exchange buffers (alternating), so one barrier
+// per pivot orders the exchange. The row interchanges become one permuted
+// write-back.
+template <typename T, typename LT>
+__device__ void tile_inverse_pivoted(LT &L, int *status) {
   static_assert(TB * TB == 4 * NT, "4 elements per thread");
+  static_assert(2 * sizeof(InvX<T>) <= offsetof(LT, mag), "exchange buffers overlay the tiles only");
   const int tid = threadIdx.x, lane = tid & 63;
   const int i = tid >> 3, c0 = (tid & 7) * 4;
   T ar[4], ai[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) { ar[q] = L.Ar[i][c0 + q]; ai[q] = L.Ai[i][c0 + q]; }
+  __syncthreads();   // every row is in registers before the exchange buffers cover Ar, Ai
+  InvX<T> *X = reinterpret_cast<InvX<T> *>(&L);
   bool bad = false;
   for (int j = 0; j < TB; ++j) {
-    T (*Br)[TBP] = (j & 1) ? L.Br : L.Pr;
-    T (*Bi)[TBP] = (j & 1) ? L.Bi : L.Pi;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) { Br[i][c0 + q] = ar[q]; Bi[i][c0 + q] = ai[q]; }
+    InvX<T> &B = X[j & 1];
+    lds_st4(&B.r[i][c0], ar);
+    lds_st4(&B.i[i][c0], ai);
     if (j >= c0 && j < c0 + 4) {
       const int q = j - c0;
       L.mag[j & 1][i] = (i >= j) ? fabs(ar[q]) + fabs(ai[q]) : T(-1);
@@ -662,9 +682,9 @@ __device__ void tile_inverse_pivoted(Lds<T> &L, int *status) {
     const int er = (i == j) ? piv : (i == piv ? j : i);  // row content after the swap
     if (er != i) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) { ar[q] = Br[er][c0 + q]; ai[q] = Bi[er][c0 + q]; }
+      for (int q = 0; q < 4; ++q) { ar[q] = B.r[er][c0 + q]; ai[q] = B.i[er][c0 + q]; }
     }
-    const T pr = Br[piv][j], pim = Bi[piv][j];
+    const T pr = B.r[piv][j], pim = B.i[piv][j];
     if (!(fabs(pr) + fabs(pim) > T(0)) || !isfinite(pr) || !isfinite(pim)) bad = true;
     T dr, di;
     crecip<T>(pr, pim, dr, di);
@@ -679,22 +699,22 @@ __device__ void tile_inverse_pivoted(Lds<T> &L, int *status) {
         }
       }
     } else {
-      const T fr = Br[er][j], fi = Bi[er][j];
+      const T fr = B.r[er][j], fi = B.i[er][j];
       const T gr = fr * dr - fi * di, gi = fr * di + fi * dr;  // f * d
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (c0 + q == j) { ar[q] = -gr; ai[q] = -gi; }
         else {
-          const T hr = Br[piv][c0 + q], hi = Bi[piv][c0 + q];
+          const T hr = B.r[piv][c0 + q], hi = B.i[piv][c0 + q];
           ar[q] -= gr * hr - gi * hi;
           ai[q] -= gr * hi + gi * hr;
         }
       }
     }
   }
-  // TB is even, so buffer 0 (Pr/Pi) was last read two steps ago
-#pragma unroll
-  for (int q = 0; q < 4; ++q) { L.Pr[i][c0 + q] = ar[q]; L.Pi[i][c0 + q] = ai[q]; }
+  // TB is even: buffer 0 was last read at step TB - 2, before step TB - 1's barrier
+  lds_st4(&X[0].r[i][c0], ar);
+  lds_st4(&X[0].i[i][c0], ai);
   if (tid == 0) {
     for (int c = 0; c < TB; ++c) L.colmap[c] = c;
     for (int j = TB - 1; j >= 0; --j) {
@@ -707,21 +727,24 @@ __device__ void tile_inverse_pivoted(Lds<T> &L, int *status) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int src = L.colmap[c0 + q];
-    L.Ar[i][c0 + q] = L.Pr[i][src];
-    L.Ai[i][c0 + q] = L.Pi[i][src];
+    ar[q] = X[0].r[i][src];
+    ai[q] = X[0].i[i][src];
   }
+  __syncthreads();   // every read of buffer 0 is done before Ar, Ai (which it overlays) are written
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { L.Ar[i][c0 + q] = ar[q]; L.Ai[i][c0 + q] = ai[q]; }
   if (bad) *status = 1;
   __syncthreads();
 }
 
-template <typename T, bool PIV>
-__device__ inline void tile_inverse(Lds<T> &L, int *status) {
-  if (PIV) tile_inverse_pivoted(L, status);
-  else tile_inverse_diag(L, status);
+template <typename T, bool PIV, typename LT>
+__device__ inline void tile_inverse(LT &L, int *status) {
+  if (PIV) tile_inverse_pivoted<T>(L, status);
+  else tile_inverse_diag<T>(L, status);
 }
 
-template <typename T>
-__device__ inline void store_pinv(const Lds<T> &L, T *slot) {
+template <typename T, typename LT>
+__device__ inline void store_pinv(const LT &L, T *slot) {
   // whole rows, 16-byte non-temporal stores (read by the next launch)
   const int i = threadIdx.x >> 3, c0 = (threadIdx.x & 7) * 4;
   store4_nt(slot + i * TB + c0, &L.Ar[i][c0]);
@@ -729,8 +752,8 @@ __device__ inline void store_pinv(const Lds<T> &L, T *slot) {
 }
 
 // S_{k+1}[0,0] tile from G_k[0,0] held in (Ar, Ai): T_{k+1} - l G u.
-template <typename T>
-__device__ inline void schur_next_00(const FJob<T> &J, int k, Lds<T> &L) {
+template <typename T, typename LT>
+__device__ inline void schur_next_00(const FJob<T> &J, int k, LT &L) {
   const int n = J.n;
   const size_t M = (size_t)J.m * n;
   for (int e = threadIdx.x; e < TB * TB; e += NT) {
@@ -790,9 +813,13 @@ __device__ inline void sym_ratio(const T *dsym, int np, int i, int j, T sign, T 
 //   otherwise      : X[I,J] - X[I,p] (P X[p,J])
 // The workgroup owning tile (p+1, p+1) also inverts its result for the next
 // step; at the last step the owner of (0,0) forms and inverts S_{k+1}[0,0].
+// The bound asks for at least six waves per SIMD (without it the register
+// allocation held the kernel to four). With 18.5 KB of LDS, eight complex64
+// workgroups fit a CU when a lane needs at most 64 registers; FP64 (35 KB)
+// is held to four by its LDS.
 template <typename T, bool PIV>
-__global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
-  __shared__ Lds<T> L;
+__global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
+  __shared__ LdsStep<T> L;
   int jb = 0, local = 0;
   if ((int)blockIdx.x < batch.njobs) {
     jb = blockIdx.x;
@@ -833,29 +860,36 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
   const T *Pslot = J.Pinv + (size_t)(step & 1) * 2 * TB * TB;
   T *Pnext = J.Pinv + (size_t)((step + 1) & 1) * 2 * TB * TB;
 
-  // every global read of this tile's update, issued together
+  // every global read of this tile's update, issued together. P goes to the
+  // B buffer while X[p,J] is in the A buffer (R = P X[p,J]), or to the A
+  // buffer when the tile has no X[p,J] (column p: -X[I,p] P); X[I,p] takes
+  // the B buffer once R has replaced X[p,J].
   typename Mfma<T>::v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  const bool has_a = tj != p, has_b = ti != p;
+  T (*Pbr)[TBP] = has_a ? L.Br : L.Ar;
+  T (*Pbi)[TBP] = has_a ? L.Bi : L.Ai;
+  constexpr int PER = TB * TB / NT;
+  T br[PER], bi[PER];
+  // symmetric jobs read X[p,J] (J < p) as tile (J,p), X[I,p] (I > p) as (p,I)
+  const bool mirA = p > 0 && J.dsym && tj < p, mirB = p > 0 && J.dsym && ti > p;
   if (p > 0) {
     // plain loads of the previous launch's output: all of them before any use
-    constexpr int PER = TB * TB / NT;
-    T pr[PER], pim[PER], ar[PER], ai[PER], br[PER], bi[PER];
-    const bool mirA = J.dsym && tj < p, mirB = J.dsym && ti > p;
+    T pr[PER], pim[PER], ar[PER], ai[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int e = tid + u * NT, a = e / TB, b = e % TB;
       pr[u] = Pslot[e];
       pim[u] = Pslot[TB * TB + e];
-      // symmetric jobs read X[p,J] (J < p) as tile (J,p), X[I,p] (I > p) as (p,I)
       const size_t ia = mirA ? (size_t)(tj * TB + a) * J.np + p * TB + b
                              : (size_t)(p * TB + a) * J.np + tj * TB + b;
       const size_t ib = mirB ? (size_t)(p * TB + a) * J.np + ti * TB + b
                              : (size_t)(ti * TB + a) * J.np + p * TB + b;
-      ar[u] = tj != p ? src[ia] : T(0);
-      ai[u] = tj != p ? src[np2 + ia] : T(0);
-      br[u] = ti != p ? src[ib] : T(0);
-      bi[u] = ti != p ? src[np2 + ib] : T(0);
+      ar[u] = has_a ? src[ia] : T(0);
+      ai[u] = has_a ? src[np2 + ia] : T(0);
+      br[u] = has_b ? src[ib] : T(0);
+      bi[u] = has_b ? src[np2 + ib] : T(0);
     }
-    if (ti != p && tj != p) {
+    if (has_a && has_b) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
@@ -867,36 +901,33 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int e = tid + u * NT, a = e / TB, b = e % TB;
-      L.Pr[a][b] = pr[u];
-      L.Pi[a][b] = pim[u];
+      Pbr[a][b] = pr[u];
+      Pbi[a][b] = pim[u];
       if (mirA) {  // X[p,J][b][a] = -(D[32 J + a] / D[32 p + b]) X[J,p][a][b]
         T rr, ri;
         sym_ratio(J.dsym, J.np, tj * TB + a, p * TB + b, T(-1), rr, ri);
         L.Ar[b][a] = rr * ar[u] - ri * ai[u];
         L.Ai[b][a] = rr * ai[u] + ri * ar[u];
-      } else if (tj != p) {
+      } else if (has_a) {
         L.Ar[a][b] = ar[u];
         L.Ai[a][b] = ai[u];
       }
-      if (mirB) {  // X[I,p][b][a] = (D[32 p + a] / D[32 I + b]) X[p,I][a][b]
+      if (mirB) {  // X[I,p][b][a] = (D[32 p + a] / D[32 I + b]) X[p,I][a][b], scaled now
         T rr, ri;
         sym_ratio(J.dsym, J.np, p * TB + a, ti * TB + b, T(1), rr, ri);
-        L.Br[b][a] = rr * br[u] - ri * bi[u];
-        L.Bi[b][a] = rr * bi[u] + ri * br[u];
-      } else if (ti != p) {
-        L.Br[a][b] = br[u];
-        L.Bi[a][b] = bi[u];
+        const T xr = rr * br[u] - ri * bi[u], xi = rr * bi[u] + ri * br[u];
+        br[u] = xr;
+        bi[u] = xi;
       }
     }
   } else {
     // p == 0: the tiles of S_k = T_k - l_k G_{k-1} u_{k-1}, formed on load
     for (int e = tid; e < TB * TB; e += NT) {
-      L.Pr[e / TB][e % TB] = Pslot[e];
-      L.Pi[e / TB][e % TB] = Pslot[TB * TB + e];
+      Pbr[e / TB][e % TB] = Pslot[e];
+      Pbi[e / TB][e % TB] = Pslot[TB * TB + e];
     }
-    if (tj != p) load_tile(J, k, p, src, p, tj, L.Ar, L.Ai);   // X[p, J]
-    if (ti != p) load_tile(J, k, p, src, ti, p, L.Br, L.Bi);   // X[I, p]
-    if (ti != p && tj != p) {
+    if (has_a) load_tile(J, k, p, src, p, tj, L.Ar, L.Ai);   // X[p, J]
+    if (has_a && has_b) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
@@ -907,30 +938,49 @@ __global__ void __launch_bounds__(NT) k_gj_step(FBatch<T> batch) {
       }
     }
   }
+  // X[I, p] into the B buffer: from the registers (p > 0) or formed on load
+  auto put_b = [&]() {
+    if (p > 0) {
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int e = tid + u * NT, a = e / TB, b = e % TB;
+        if (mirB) {  // already scaled: transposed into place
+          L.Br[b][a] = br[u];
+          L.Bi[b][a] = bi[u];
+        } else {
+          L.Br[a][b] = br[u];
+          L.Bi[a][b] = bi[u];
+        }
+      }
+    } else {
+      load_tile(J, k, p, src, ti, p, L.Br, L.Bi);
+    }
+  };
+  if (!has_a && has_b) put_b();   // column p: P is in the A buffer
   __syncthreads();
-  if (tj != p) {  // R = P X[p, J]
+  if (has_a) {  // R = P X[p, J]
     typename Mfma<T>::v4 rr = {0, 0, 0, 0}, ri = {0, 0, 0, 0};
-    cgemm_quad<T>(rr, ri, L.Pr, L.Pi, L.Ar, L.Ai, qi, qj, lane, T(1));
-    __syncthreads();  // every wave has read X[p, J]; R takes its place
+    cgemm_quad<T>(rr, ri, L.Br, L.Bi, L.Ar, L.Ai, qi, qj, lane, T(1));
+    __syncthreads();  // every wave has read P and X[p, J]: R and X[I, p] take their places
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
       L.Ar[a][b] = rr[t];
       L.Ai[a][b] = ri[t];
     }
-    if (ti == p) { cr = rr; ci = ri; }
+    if (has_b) put_b();
+    else { cr = rr; ci = ri; }
     __syncthreads();
   }
-  if (ti == p && tj == p) {
+  if (!has_a && !has_b) {  // (p, p): P, in the A buffer
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int a = qi * 16 + Mfma<T>::crow(lane, t), b = qj * 16 + (lane & 15);
-      cr[t] = L.Pr[a][b];
-      ci[t] = L.Pi[a][b];
+      cr[t] = L.Ar[a][b];
+      ci[t] = L.Ai[a][b];
     }
-  } else if (ti != p) {
-    if (tj == p) cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Pr, L.Pi, qi, qj, lane, T(-1));
-    else cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Ar, L.Ai, qi, qj, lane, T(-1));
+  } else if (has_b) {  // -X[I,p] P (column p) or X[I,J] - X[I,p] R; the right factor is in the A buffer
+    cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Ar, L.Ai, qi, qj, lane, T(-1));
   }
   // the tile goes out through LDS as whole rows (128-byte lines, 16-byte
   // stores) that bypass this XCD's L2: the next launch reads it from memory,
