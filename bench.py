@@ -173,49 +173,62 @@ def _batched_factor(ops, precision, stream_timed=True):
 def roofline_leg(n=ROOF_N, precision="c64", njobs=ROOF_JOBS, traffic=None):
     """The dominant kernel of the stream, k_gj_step, at the stream's batching.
 
-    njobs factorizations of n x n operators share each step launch
-    (fdfd_factor_batch: n*(n/32) + 1 launches), timed with HIP events on
-    their stream; the x-wrap corrections that follow them are timed alone
-    and taken out. Algorithmic flops per launch: njobs * 8 n^2 32 (the
-    8 m n^3 of one block factorization over its n*n/32 launches). The same
-    factorization with one job per launch (the launch-per-step path of a
-    single request) and the persistent single-launch factor are timed too.
+    njobs twisted factorizations of n x n operators share each step launch
+    (fdfd_factor_batch: every launch carries a step of each operator's top
+    chain and, until it ends, of its bottom chain; (c + 1) n/32 + 1 launches
+    for twist block c), timed with HIP events on their stream; the x-wrap
+    corrections that follow them are timed alone and taken out. Algorithmic
+    flops: 8 m n^3 per factorization (m = n blocks), spread over the
+    launches. One factorization alone (its two chains per launch) is timed
+    too, and, for comparison, the one-chain factorization (stepwise and the
+    persistent single-launch kernel), both with their wrap corrections.
     """
+    import ctypes
+    from fdfdpy_amd import _lib as L
+    lib = L.lib()
     ops = _roof_ops(n, njobs)
+    c = lib.fdfd_twist_block(ctypes.byref(ops[0].grid))
     ms_all = _batched_factor(ops, precision)
     ms_wrap = sum(_timed_wrap(op, precision) for op in ops)
     ms = ms_all - ms_wrap
     one = ops[0]
     ms1 = _timed_factor(one, precision, stepwise=True) - ms_wrap / njobs
-    ms_persist = _timed_factor(one, precision, stepwise=False) - ms_wrap / njobs
+    ms_single = _timed_factor(one, precision, stepwise=False)          # with its wrap correction
+    prev = lib.fdfd_set_twist(0)
+    try:
+        ms_chain = _timed_factor(one, precision, stepwise=True)        # one chain, with its wrap correction
+        ms_persist = _timed_factor(one, precision, stepwise=False)
+        one.drop_factor()
+    finally:
+        lib.fdfd_set_twist(prev)
     nt = (n + 31) // 32
-    launches = n * nt + 1
-    flops_launch = 8.0 * n * n * 32
-    flops = 8.0 * n ** 4                       # 8 m n^3 real flops, m = n blocks
+    launches = (c + 1) * nt + 1
+    flops = 8.0 * n ** 4                       # 8 m n^3 real flops per factorization, m = n blocks
     avg_us = ms * 1e3 / launches
-    achieved = njobs * flops_launch / (avg_us * 1e-6) / 1e12
+    achieved = njobs * flops / (ms * 1e-3) / 1e12
     peak = FP32_MFMA_PEAK_TFLOPS if precision == "c64" else FP64_MFMA_PEAK_TFLOPS
     tname = "float" if precision == "c64" else "double"
     out = {
         "kernel": f"k_gj_step<{tname}, false>",
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
         "frac": round(achieved / peak, 5), "traffic": None,
-        "workload": f"{njobs} complex factors of {n}x{n} Ez operators sharing each step launch: {launches} "
-                    f"launches of {njobs} x 8*n^2*32 = {njobs * flops_launch:.3e} flops",
+        "workload": f"{njobs} twisted complex factors of {n}x{n} Ez operators (twist block {c}) sharing "
+                    f"{launches} step launches: {njobs * flops:.3e} flops, {njobs * flops / launches:.3e} per launch",
         "avg_launch_us": round(avg_us, 3), "batched_factor_ms": round(ms, 3), "wrap_ms": round(ms_wrap, 3),
         "hbm_gbs_algorithmic": None,
-        "one_job_per_launch": {"avg_launch_us": round(ms1 * 1e3 / launches, 3), "factor_ms": round(ms1, 3),
-                               "achieved_tflops": round(flops / (ms1 * 1e-3) / 1e12, 3)},
-        "persistent_factor_ms": round(ms_persist, 3),
-        "persistent_achieved_tflops": round(flops / (ms_persist * 1e-3) / 1e12, 3),
+        "one_request": {"avg_launch_us": round(ms1 * 1e3 / launches, 3), "factor_ms": round(ms1, 3),
+                        "achieved_tflops": round(flops / (ms1 * 1e-3) / 1e12, 3),
+                        "factor_with_wrap_ms": round(ms_single, 3)},
+        "one_chain": {"stepwise_with_wrap_ms": round(ms_chain, 3), "persistent_with_wrap_ms": round(ms_persist, 3)},
     }
-    # algorithmic bytes per launch: the upper tiles of every job's working
-    # matrix read and written once (symmetric steps), plus the tiles below the
-    # diagonal written at each block's last step
+    # algorithmic bytes: per block step, the upper tiles of the working matrix
+    # read and written once (symmetric steps), plus the tiles below the
+    # diagonal written at each block's last step; n blocks of nt steps per
+    # factorization, over the launches
     es = 8 if precision == "c64" else 16
     upper = nt * (nt + 1) // 2 * 32 * 32 * es * 2
     lower = nt * (nt - 1) // 2 * 32 * 32 * es / nt
-    alg = njobs * (upper + lower)
+    alg = njobs * n * nt * (upper + lower) / launches
     out["algorithmic_bytes_per_launch"] = round(alg)
     out["hbm_gbs_algorithmic"] = round(alg / (avg_us * 1e-6) / 1e9, 1)
     if traffic:

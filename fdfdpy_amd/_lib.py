@@ -87,6 +87,8 @@ SIGNATURES = {
     "fdfd_engine_launches": (ctypes.c_uint64, [c_void_p]),
     "fdfd_set_spin_cap": (None, [ctypes.c_uint]),
     "fdfd_set_step_symmetry": (c_int, [c_int]),
+    "fdfd_set_twist": (c_int, [c_int]),
+    "fdfd_twist_block": (c_int, [_G]),
 }
 
 _lib = None

@@ -14,6 +14,8 @@ HIPCC_FLAGS = [
     # no packed-FP32 instructions anywhere in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
+    # the FP64 step kernel cannot reach the six-wave bound (its LDS holds it to four)
+    "-Wno-pass-failed",
     "-I", os.path.join(ROOT, "include"),
 ]
 

@@ -36,7 +36,8 @@ namespace {
 constexpr int TB = 32;        // pivot tile / output tile edge
 constexpr int TBP = TB + 1;   // padded LDS row
 constexpr int NT = 256;       // threads per workgroup (4 waves)
-constexpr int MAXJ = 16;      // jobs per batched step launch
+constexpr int MAXJ = 32;      // factorization chains per batched step launch
+constexpr int MAX_BATCH = 16; // operators per fdfd_factor_batch call (a twisted one is two chains)
 constexpr int MAXR = 4;       // right-hand sides per sweep pass (per-block sweeps)
 constexpr int MAX_BLOCK_N = 2048;   // largest block (the shorter grid side)
 constexpr int NORM_PARTS = 512;
@@ -252,6 +253,7 @@ constexpr unsigned SPIN_CAP = 1u << 21;
 // host settings, read by the calling threads and the engines' scheduler threads
 std::atomic<unsigned> g_spin_cap{SPIN_CAP};  // passed to the persistent kernels
 std::atomic<int> g_step_sym{1};              // launch-per-step factorizations use symmetric steps
+std::atomic<int> g_twist{1};                 // truncated-wrap factorizations are twisted where they fit
 
 template <typename T>
 __device__ inline T ld_xcd(const T *p) {
@@ -318,6 +320,9 @@ template <typename T> struct FJob {
                      // by the initial-pivot launch; null: every tile of every step
   int m, n, np, nt;  // blocks, block size, padded block size, tiles per edge
   int k, p;          // step performed by this launch; k < 0: initial pivot
+  int dir;           // 1: blocks upwards from 0 (the whole chain, or the top chain of a twisted
+                     // factorization, ending with its twist block c); -1: the bottom chain m-1..c+1
+  int c;             // the twist block (m - 1: not twisted)
 };
 // Workgroup b < njobs takes the critical tile of job b (the tile inverted for
 // the next step), so those start first; the rest of the first crow * XCDS
@@ -390,7 +395,10 @@ __device__ inline void store4_nt(double *p, const double *v) {
 }
 
 // Element (a, b) of the working matrix of block k before step p. For p == 0
-// this is S_k = T_k - l_k G_{k-1} u_{k-1} (identity on the padding).
+// this is the Schur complement of block k (identity on the padding):
+// S_k = T_k - XM_k G_{k-1} XP_{k-1} on the top chain,
+// S'_k = T_k - XP_k G_{k+1} XM_{k+1} on the bottom chain, both terms at the
+// twist block.
 template <typename T>
 __device__ inline void x_elem(const FJob<T> &J, int k, int p, const T *src, int a, int b,
                               T &re, T &im) {
@@ -417,12 +425,23 @@ __device__ inline void x_elem(const FJob<T> &J, int k, int p, const T *src, int 
   if (a == b) { cd c = ld_c(J.op, FDFD_OP_C * M + pa); vr += c.r; vi += c.i; }
   if (b == (a + 1) % n) { cd c = ld_c(J.op, FDFD_OP_YP * M + pa); vr += c.r; vi += c.i; }
   if (b == (a + n - 1) % n) { cd c = ld_c(J.op, FDFD_OP_YM * M + pa); vr += c.r; vi += c.i; }
-  if (k > 0) {
+  // the eliminated neighbours: the block above (top chain), the block
+  // below (bottom chain, and the twist block of the top chain)
+  if (J.dir > 0 && k > 0) {
     const T *Gp = J.G + (size_t)(k - 1) * 2 * np2;
     cd gv = {(double)Gp[(size_t)a * J.np + b], (double)Gp[np2 + (size_t)a * J.np + b]};
     cd l = ld_c(J.op, FDFD_OP_XM * M + pa);
     cd u = ld_c(J.op, FDFD_OP_XP * M + (size_t)(k - 1) * n + b);
     cd s = cmul(cmul(l, gv), u);
+    vr -= s.r;
+    vi -= s.i;
+  }
+  if (J.dir < 0 ? k + 1 < J.m : (k == J.c && J.c < J.m - 1)) {
+    const T *Gn = J.G + (size_t)(k + 1) * 2 * np2;
+    cd gv = {(double)Gn[(size_t)a * J.np + b], (double)Gn[np2 + (size_t)a * J.np + b]};
+    cd u = ld_c(J.op, FDFD_OP_XP * M + pa);
+    cd l = ld_c(J.op, FDFD_OP_XM * M + (size_t)(k + 1) * n + b);
+    cd s = cmul(cmul(u, gv), l);
     vr -= s.r;
     vi -= s.i;
   }
@@ -751,25 +770,37 @@ __device__ inline void store_pinv(const LT &L, T *slot) {
   store4_nt(slot + TB * TB + i * TB + c0, &L.Ai[i][c0]);
 }
 
-// S_{k+1}[0,0] tile from G_k[0,0] held in (Ar, Ai): T_{k+1} - l G u.
+// The [0,0] tile of the chain's next block kn = k + dir from G_k[0,0] held
+// in (Ar, Ai): T_kn - XM_kn G_k XP_k (top chain) or T_kn - XP_kn G_k XM_k
+// (bottom chain); at the twist block also - XP_kn G_{kn+1} XM_{kn+1}, from the
+// bottom chain's last block, finished in an earlier launch.
 template <typename T, typename LT>
 __device__ inline void schur_next_00(const FJob<T> &J, int k, LT &L) {
-  const int n = J.n;
-  const size_t M = (size_t)J.m * n;
+  const int n = J.n, kn = k + J.dir;
+  const size_t M = (size_t)J.m * n, np2 = (size_t)J.np * J.np;
+  const bool twist = J.dir > 0 && kn == J.c && J.c < J.m - 1;
+  const T *Gb = J.G + (size_t)(kn + 1) * 2 * np2;   // read only at the twist block
   for (int e = threadIdx.x; e < TB * TB; e += NT) {
     const int a = e / TB, b = e % TB;
     double vr, vi;
     if (a >= n || b >= n) { vr = (a == b) ? 1.0 : 0.0; vi = 0.0; }
     else {
-      const size_t pa = (size_t)(k + 1) * n + a;
+      const size_t pa = (size_t)kn * n + a;
       vr = 0; vi = 0;
       if (a == b) { cd c = ld_c(J.op, FDFD_OP_C * M + pa); vr += c.r; vi += c.i; }
       if (b == (a + 1) % n) { cd c = ld_c(J.op, FDFD_OP_YP * M + pa); vr += c.r; vi += c.i; }
       if (b == (a + n - 1) % n) { cd c = ld_c(J.op, FDFD_OP_YM * M + pa); vr += c.r; vi += c.i; }
       cd gv = {(double)L.Ar[a][b], (double)L.Ai[a][b]};
-      cd s = cmul(cmul(ld_c(J.op, FDFD_OP_XM * M + pa), gv),
-                  ld_c(J.op, FDFD_OP_XP * M + (size_t)k * n + b));
+      const cd s = J.dir > 0
+          ? cmul(cmul(ld_c(J.op, FDFD_OP_XM * M + pa), gv), ld_c(J.op, FDFD_OP_XP * M + (size_t)k * n + b))
+          : cmul(cmul(ld_c(J.op, FDFD_OP_XP * M + pa), gv), ld_c(J.op, FDFD_OP_XM * M + (size_t)k * n + b));
       vr -= s.r; vi -= s.i;
+      if (twist) {
+        cd g2 = {(double)Gb[(size_t)a * J.np + b], (double)Gb[np2 + (size_t)a * J.np + b]};
+        const cd s2 = cmul(cmul(ld_c(J.op, FDFD_OP_XP * M + pa), g2),
+                           ld_c(J.op, FDFD_OP_XM * M + (size_t)(kn + 1) * n + b));
+        vr -= s2.r; vi -= s2.i;
+      }
     }
     L.Ar[a][b] = (T)vr;  // each (a,b) read and written by the same thread
     L.Ai[a][b] = (T)vi;
@@ -841,10 +872,10 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   const size_t np2 = (size_t)J.np * J.np;
   const int nt = J.nt;
 
-  if (J.k < 0) {  // initial pivot: inverse of T_0[0,0]
+  if (J.k < 0) {  // initial pivot: inverse of T_k0[0,0], k0 the chain's first block
     if (local == 0) {
-      if (J.dsym) block_symmetrizer(J);
-      load_tile(J, 0, 0, (const T *)nullptr, 0, 0, L.Ar, L.Ai);
+      if (J.dsym && J.dir > 0) block_symmetrizer(J);
+      load_tile(J, J.dir > 0 ? 0 : J.m - 1, 0, (const T *)nullptr, 0, 0, L.Ar, L.Ai);
       __syncthreads();
       tile_inverse<T, PIV>(L, J.status);
       store_pinv(L, J.Pinv);  // slot of global step 0
@@ -856,7 +887,7 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   T *Gk = J.G + (size_t)k * 2 * np2;
   const T *src = (p == 0) ? nullptr : (((nt - p) % 2 == 0) ? Gk : J.W);
   T *dst = ((nt - 1 - p) % 2 == 0) ? Gk : J.W;
-  const long step = (long)k * nt + p;
+  const long step = (long)(J.dir > 0 ? k : J.m - 1 - k) * nt + p;   // of this chain
   const T *Pslot = J.Pinv + (size_t)(step & 1) * 2 * TB * TB;
   T *Pnext = J.Pinv + (size_t)((step + 1) & 1) * 2 * TB * TB;
 
@@ -994,7 +1025,7 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   }
   __syncthreads();
   const bool next_pivot = (p + 1 < nt) && ti == p + 1 && tj == p + 1;
-  const bool next_block = (p == nt - 1) && ti == 0 && tj == 0 && k + 1 < J.m;
+  const bool next_block = (p == nt - 1) && ti == 0 && tj == 0 && (J.dir > 0 ? k < J.c : k > J.c + 1);
   // the tile of the next pivot is replaced by its inverse at the next step
   // and read by no workgroup before that: its store is skipped
   if (!next_pivot) {
@@ -1231,11 +1262,15 @@ __global__ void __launch_bounds__(NT) k_gj_persist(PJob<T> J) {
 }
 
 // ---------------------------------------------------------------- sweeps
-// Forward: y_k = G_k (r_k - l_k y_{k-1}); y is [nrhs][nx][np] complex T.
+// Forward, one block: y_k = G_k (r_k - p1(k) y_nb1 - p2(k) y_nb2) for the
+// neighbours nb1, nb2 >= 0 (band planes p1, p2; -1: none); y is
+// [nrhs][nx][np] complex T. With x (the twist block of a twisted
+// factorization, whose y is already its solution) x_k += y_k (FP64).
 template <typename T>
 __global__ void __launch_bounds__(NT) k_sweep_fwd(const T *__restrict__ G, const double *__restrict__ op,
                                                   const double *__restrict__ r, T *__restrict__ y,
-                                                  int nx, int ny, int np, int k, int nrhs) {
+                                                  double *__restrict__ x, int nx, int ny, int np, int k,
+                                                  int nrhs, int nb1, int p1, int nb2, int p2) {
   extern __shared__ unsigned char sweep_lds[];
   T *vr = reinterpret_cast<T *>(sweep_lds), *vi = vr + (size_t)MAXR * np;   // [MAXR][np] each
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -1249,9 +1284,13 @@ __global__ void __launch_bounds__(NT) k_sweep_fwd(const T *__restrict__ G, const
       double xr = 0, xi = 0;
       if (c < ny) {
         cd v = ld_c(r, (size_t)(r0 + rr) * M + (size_t)k * ny + c);
-        if (k > 0) {
-          const T *yp = y + 2 * (((size_t)(r0 + rr) * nx + (k - 1)) * np + c);
-          v = csub(v, cmul(ld_c(op, FDFD_OP_XM * M + (size_t)k * ny + c), cd{(double)yp[0], (double)yp[1]}));
+        if (nb1 >= 0) {
+          const T *yp = y + 2 * (((size_t)(r0 + rr) * nx + nb1) * np + c);
+          v = csub(v, cmul(ld_c(op, (size_t)p1 * M + (size_t)k * ny + c), cd{(double)yp[0], (double)yp[1]}));
+        }
+        if (nb2 >= 0) {
+          const T *yp = y + 2 * (((size_t)(r0 + rr) * nx + nb2) * np + c);
+          v = csub(v, cmul(ld_c(op, (size_t)p2 * M + (size_t)k * ny + c), cd{(double)yp[0], (double)yp[1]}));
         }
         xr = v.r; xi = v.i;
       }
@@ -1285,6 +1324,11 @@ __global__ void __launch_bounds__(NT) k_sweep_fwd(const T *__restrict__ G, const
           T *yo = y + 2 * (((size_t)(r0 + q) * nx + k) * np + a);
           yo[0] = ar[q];
           yo[1] = ai[q];
+          if (x && a < ny) {
+            double *xo = x + 2 * ((size_t)(r0 + q) * M + (size_t)k * ny + a);
+            xo[0] += (double)ar[q];
+            xo[1] += (double)ai[q];
+          }
         }
       }
     }
@@ -1292,11 +1336,12 @@ __global__ void __launch_bounds__(NT) k_sweep_fwd(const T *__restrict__ G, const
   }
 }
 
-// Backward: d_k = y_k - G_k (u_k d_{k+1}) in place of y; x_k += d_k (FP64).
+// Backward, one block: d_k = y_k - G_k (p(k) d_nb) in place of y (d_k = y_k
+// for nb < 0); x_k += d_k (FP64).
 template <typename T>
 __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const double *__restrict__ op,
                                                   T *__restrict__ y, double *__restrict__ x,
-                                                  int nx, int ny, int np, int k, int nrhs) {
+                                                  int nx, int ny, int np, int k, int nrhs, int nb, int pl) {
   extern __shared__ unsigned char sweep_lds[];
   T *vr = reinterpret_cast<T *>(sweep_lds), *vi = vr + (size_t)MAXR * np;   // [MAXR][np] each
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -1308,9 +1353,9 @@ __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const
     for (int e = tid; e < nr * np; e += NT) {
       const int rr = e / np, c = e % np;
       double xr = 0, xi = 0;
-      if (c < ny && k + 1 < nx) {
-        const T *dn = y + 2 * (((size_t)(r0 + rr) * nx + (k + 1)) * np + c);
-        cd v = cmul(ld_c(op, FDFD_OP_XP * M + (size_t)k * ny + c), cd{(double)dn[0], (double)dn[1]});
+      if (c < ny && nb >= 0) {
+        const T *dn = y + 2 * (((size_t)(r0 + rr) * nx + nb) * np + c);
+        cd v = cmul(ld_c(op, (size_t)pl * M + (size_t)k * ny + c), cd{(double)dn[0], (double)dn[1]});
         xr = v.r; xi = v.i;
       }
       vr[rr * np + c] = (T)xr;
@@ -1321,7 +1366,7 @@ __global__ void __launch_bounds__(NT) k_sweep_bwd(const T *__restrict__ G, const
       T ar[MAXR], ai[MAXR];
 #pragma unroll
       for (int q = 0; q < MAXR; ++q) { ar[q] = 0; ai[q] = 0; }
-      if (k + 1 < nx) {
+      if (nb >= 0) {
         for (int c = lane; c < np; c += 64) {
           const T gr = Gk[(size_t)a * np + c], gi = Gk[np2 + (size_t)a * np + c];
 #pragma unroll
@@ -1401,14 +1446,38 @@ inline int pad64(int n) { return (n + 63) / 64 * 64; }
 typedef unsigned long long u64;
 constexpr int MAXW = PSWEEP_MAX_N / (4 * SRW);   // sweep workgroups at the largest block
 constexpr int LL_MAXRHS = 1;                      // right-hand sides of the tagged-word sweep
-struct SweepSync {
-  unsigned abort;
-  unsigned pad[63];
+// Exchange state of one block chain of a persistent sweep
+struct SweepChainSync {
   unsigned done[2][MAXW];   // flag sweep, forward / backward: steps finished per workgroup
   // flag sweep: the next block's input vector, [2 parity][RP][np] complex
   double vbuf[2 * MAXRHS * PSWEEP_MAX_N];
   // tagged-word sweep: [2 parity][RP][np][re/im][words per component]
   u64 llbuf[2 * LL_MAXRHS * PSWEEP_MAX_N * 2 * 2];
+};
+struct SweepSync {
+  unsigned abort;
+  unsigned pad[63];
+  SweepChainSync ch[2];
+};
+
+// The block chains of one persistent sweep launch. The factorization is
+// either one chain over blocks 0..m-1 or twisted: a top chain 0..c-1, a
+// bottom chain m-1..c+1 and the twist block c between them (solved by its
+// own launch). Chain i runs on workgroups [i W, (i + 1) W) and visits ns
+// blocks k0, k0 + dk, ...; cpl is the band plane of the coupling between a
+// block and the chain's previous block, read at the next block
+// (forward: v_kn = r_kn - cpl(kn) y_k; backward: w_kn = cpl(kn) d_k). The
+// first block's input is r_k0 (forward), zero (backward of a single chain:
+// block m-1 has no successor) or cpl(k0) y_t (backward of a twisted chain:
+// its coupling to the twist block t).
+struct SweepChain {
+  int k0, dk, ns, cpl;
+  int from_twist;   // backward: the first input is cpl(k0) y_t
+  int t;            // the twist block
+};
+struct SweepChains {
+  int nch;
+  SweepChain ch[2];
 };
 
 // Sums v[0..N) over the 64 lanes of a wave at once: each round halves the
@@ -1471,17 +1540,19 @@ __device__ inline void ld_xcd_c(const double *p, double &re, double &im) {
 template <typename T, int RP>
 __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const double *__restrict__ op,
                                                const double *__restrict__ r, T *y, double *__restrict__ x,
-                                               int nx, int ny, int np, int nrhs, int dir, SweepSync *sy,
-                                               unsigned cap) {
+                                               int nx, int ny, int np, int nrhs, int dir, SweepChains chs,
+                                               SweepSync *sy, unsigned cap) {
   extern __shared__ unsigned char smem_raw[];
   const int nv = (np + 63) / 64 * 64;                  // padded row length of v
   T *vr = reinterpret_cast<T *>(smem_raw);             // [RP][nv]
   T *vi = vr + (size_t)RP * nv;
-  T *vb = reinterpret_cast<T *>(sy->vbuf);             // [2][RP][np][2]
+  const int W = gridDim.x / chs.nch, grp = blockIdx.x / W, wl = blockIdx.x % W;
+  const SweepChain ch = chs.ch[grp];
+  SweepChainSync *cs = &sy->ch[grp];
+  T *vb = reinterpret_cast<T *>(cs->vbuf);             // [2][RP][np][2]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const size_t np2 = (size_t)np * np, M = (size_t)nx * ny;
-  const int W = gridDim.x;
-  const int row0 = (blockIdx.x * 4 + wv) * SRW;
+  const int row0 = (wl * 4 + wv) * SRW;
   // the (row, rhs, re/im) value this lane finishes after the reduction
   static_assert(RP >= 1 && RP <= MAXRHS && (RP & (RP - 1)) == 0, "RP must be a power of two");
   constexpr int NV = 2 * SRW * RP, SH = 6 - ilog2(NV);
@@ -1490,14 +1561,14 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
   const int my_a = row0 + my_rr;
   const bool my_store = (lane & ((1 << SH) - 1)) == 0 && my_q < nrhs && my_a < np;
   const bool my_real = my_q < nrhs && my_a < ny;
-  const int cpl_plane = dir == 0 ? FDFD_OP_XM : FDFD_OP_XP;
-  for (int s = 0; s < nx; ++s) {
-    const int k = dir == 0 ? s : nx - 1 - s;
+  const int cpl_plane = ch.cpl;
+  for (int s = 0; s < ch.ns; ++s) {
+    const int k = ch.k0 + ch.dk * s;
     const T *Gk = G + (size_t)k * 2 * np2;
-    const bool coupled = dir == 0 ? k > 0 : k + 1 < nx;
-    const bool active = dir == 0 || coupled;
-    const int kn = dir == 0 ? k + 1 : k - 1;             // the next block
-    const bool has_next = dir == 0 ? k + 1 < nx : k > 0;
+    // the first backward block of a single chain has no successor: d = y
+    const bool active = dir == 0 || s > 0 || ch.from_twist;
+    const int kn = k + ch.dk;                            // the next block
+    const bool has_next = s + 1 < ch.ns;
     // 1. this workgroup's rows of G_k and the inputs of its published
     //    entries (independent of the previous block)
     T gr[SRW][MAXCL], gi[SRW][MAXCL];
@@ -1512,7 +1583,7 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
         gi[rr][cc] = ok ? Gk[np2 + (size_t)a * np + c] : T(0);
       }
     }
-    cd rn = {0, 0}, cn = {0, 0};   // forward: r_{k+1}, XM_{k+1}; backward: XP_{k-1} (this lane's row)
+    cd rn = {0, 0}, cn = {0, 0};   // forward: r_kn, cpl(kn); backward: cpl(kn) (this lane's row)
     if (my_real && has_next) {
       if (dir == 0) rn = ld_c(r, (size_t)my_q * M + (size_t)kn * ny + my_a);
       cn = ld_c(op, cpl_plane * M + (size_t)kn * ny + my_a);
@@ -1520,9 +1591,9 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
     T yown = 0;   // backward: this lane's entry of y_k, written by the forward launch
     T *my_y = y + 2 * (((size_t)my_q * nx + k) * np + my_a) + my_ri;
     if (dir == 1 && my_store) yown = *my_y;
-    // 2. this block's input vector into LDS: r_0 at the first forward step,
-    //    zero at the first backward step, else the vector all workgroups
-    //    published at the previous step (every load issued before any use)
+    // 2. this block's input vector into LDS: the chain's first input at its
+    //    first step, else the vector all workgroups published at the previous
+    //    step (every load issued before any use)
     if (s == 0) {
       constexpr int CU = (PSWEEP_MAX_N + NT - 1) / NT;
       double ar[RP][CU], ai[RP][CU];
@@ -1531,10 +1602,16 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
 #pragma unroll
         for (int u = 0; u < CU; ++u) {
           const int c = tid + u * NT;
-          const bool ok = dir == 0 && q < nrhs && c < ny;
-          const double *rp = r + 2 * ((size_t)q * M + (size_t)k * ny + c);
-          ar[q][u] = ok ? rp[0] : 0.0;
-          ai[q][u] = ok ? rp[1] : 0.0;
+          const bool ok = q < nrhs && c < ny;
+          cd v = {0, 0};
+          if (ok && dir == 0) {
+            v = ld_c(r, (size_t)q * M + (size_t)k * ny + c);
+          } else if (ok && ch.from_twist) {
+            const T *yt = y + 2 * (((size_t)q * nx + ch.t) * np + c);
+            v = cmul(ld_c(op, (size_t)cpl_plane * M + (size_t)k * ny + c), cd{(double)yt[0], (double)yt[1]});
+          }
+          ar[q][u] = v.r;
+          ai[q][u] = v.i;
         }
 #pragma unroll
       for (int q = 0; q < RP; ++q)
@@ -1548,7 +1625,7 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
         }
       __syncthreads();
     } else {
-      wait_all_flags(sy->done[dir], W, (unsigned)s, &sy->abort, cap);
+      wait_all_flags(cs->done[dir], W, (unsigned)s, &sy->abort, cap);
       const T *vin = vb + (size_t)(s & 1) * RP * np * 2;
       constexpr int CU = (PSWEEP_MAX_N + NT - 1) / NT;
       T br[RP][CU], bi[RP][CU];
@@ -1603,7 +1680,7 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
       else if (my_a < ny) x[2 * ((size_t)my_q * M + (size_t)k * ny + my_a) + my_ri] += (double)out;
     }
     // 5. publish the next block's input entry for this row:
-    //    forward v_{k+1} = r_{k+1} - XM_{k+1} y_k, backward w_{k-1} = XP_{k-1} d_k
+    //    forward v_kn = r_kn - cpl(kn) y_k, backward w_kn = cpl(kn) d_k
     const T other = __shfl_xor(out, 1 << SH);   // the other component of this (row, rhs)
     if (my_store && has_next) {
       const double ore = (double)(my_ri == 0 ? out : other), oim = (double)(my_ri == 0 ? other : out);
@@ -1616,7 +1693,7 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
     stores_complete();
     __syncthreads();
     if (tid == 0)
-      __hip_atomic_store(&sy->done[dir][blockIdx.x], (unsigned)(s + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&cs->done[dir][wl], (unsigned)(s + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -1624,17 +1701,20 @@ __global__ void __launch_bounds__(NT) k_psweep(const T *__restrict__ G, const do
 template <typename T, int RP>
 __global__ void __launch_bounds__(NT) k_psweep_ll(const T *__restrict__ G, const double *__restrict__ op,
                                                   const double *__restrict__ r, T *y, double *__restrict__ x,
-                                                  int nx, int ny, int np, int nrhs, int dir, SweepSync *sy,
-                                                  unsigned cap, unsigned tag0) {
+                                                  int nx, int ny, int np, int nrhs, int dir, SweepChains chs,
+                                                  SweepSync *sy, unsigned cap, unsigned tag0) {
   extern __shared__ unsigned char smem_raw[];
   const int nv = (np + 63) / 64 * 64;                  // padded row length of v
   T *vr = reinterpret_cast<T *>(smem_raw);             // [RP][nv]
   T *vi = vr + (size_t)RP * nv;
+  const int W = gridDim.x / chs.nch, grp = blockIdx.x / W, wl = blockIdx.x % W;
+  const SweepChain ch = chs.ch[grp];
+  SweepChainSync *cs = &sy->ch[grp];
   constexpr int WPC = sweep_wpc<T>();
   const size_t vstride = (size_t)RP * np * 2 * WPC;    // words of one parity buffer
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const size_t np2 = (size_t)np * np, M = (size_t)nx * ny;
-  const int row0 = (blockIdx.x * 4 + wv) * SRW;
+  const int row0 = (wl * 4 + wv) * SRW;
   // the (row, rhs, re/im) value this lane finishes after the reduction
   static_assert(RP >= 1 && RP <= MAXRHS && (RP & (RP - 1)) == 0, "RP must be a power of two");
   constexpr int NV = 2 * SRW * RP, SH = 6 - ilog2(NV);
@@ -1643,7 +1723,7 @@ __global__ void __launch_bounds__(NT) k_psweep_ll(const T *__restrict__ G, const
   const int my_a = row0 + my_rr;
   const bool my_store = (lane & ((1 << SH) - 1)) == 0 && my_q < nrhs && my_a < np;
   const bool my_real = my_q < nrhs && my_a < ny;
-  const int cpl_plane = dir == 0 ? FDFD_OP_XM : FDFD_OP_XP;
+  const int cpl_plane = ch.cpl;
   // words this thread receives per step: component c of rhs q is word
   // ((q * np + c) * 2 + re/im) * WPC + half
   constexpr int CW = (RP * PSWEEP_MAX_N * 2 * WPC + NT - 1) / NT;
@@ -1665,13 +1745,13 @@ __global__ void __launch_bounds__(NT) k_psweep_ll(const T *__restrict__ G, const
     vr[e] = T(0);
     vi[e] = T(0);
   }
-  for (int s = 0; s < nx; ++s) {
-    const int k = dir == 0 ? s : nx - 1 - s;
+  for (int s = 0; s < ch.ns; ++s) {
+    const int k = ch.k0 + ch.dk * s;
     const T *Gk = G + (size_t)k * 2 * np2;
-    const bool coupled = dir == 0 ? k > 0 : k + 1 < nx;
-    const bool active = dir == 0 || coupled;
-    const int kn = dir == 0 ? k + 1 : k - 1;             // the next block
-    const bool has_next = dir == 0 ? k + 1 < nx : k > 0;
+    // the first backward block of a single chain has no successor: d = y
+    const bool active = dir == 0 || s > 0 || ch.from_twist;
+    const int kn = k + ch.dk;                            // the next block
+    const bool has_next = s + 1 < ch.ns;
     // 1. this workgroup's rows of G_k and the inputs of its published
     //    entries (independent of the previous block)
     T gr[SRW][MAXCL], gi[SRW][MAXCL];
@@ -1686,7 +1766,7 @@ __global__ void __launch_bounds__(NT) k_psweep_ll(const T *__restrict__ G, const
         gi[rr][cc] = ok ? Gk[np2 + (size_t)a * np + c] : T(0);
       }
     }
-    cd rn = {0, 0}, cn = {0, 0};   // forward: r_{k+1}, XM_{k+1}; backward: XP_{k-1} (this lane's row)
+    cd rn = {0, 0}, cn = {0, 0};   // forward: r_kn, cpl(kn); backward: cpl(kn) (this lane's row)
     if (my_real && has_next) {
       if (dir == 0) rn = ld_c(r, (size_t)my_q * M + (size_t)kn * ny + my_a);
       cn = ld_c(op, cpl_plane * M + (size_t)kn * ny + my_a);
@@ -1694,9 +1774,9 @@ __global__ void __launch_bounds__(NT) k_psweep_ll(const T *__restrict__ G, const
     T yown = 0;   // backward: this lane's entry of y_k, written by the forward launch
     T *my_y = y + 2 * (((size_t)my_q * nx + k) * np + my_a) + my_ri;
     if (dir == 1 && my_store) yown = *my_y;
-    // 2. this block's input vector into LDS: r_0 at the first forward step,
-    //    zero at the first backward step, else the vector all workgroups
-    //    published at the previous step
+    // 2. this block's input vector into LDS: the chain's first input at its
+    //    first step, else the vector all workgroups published at the previous
+    //    step
     __syncthreads();   // every wave has finished reading the previous block's vector
     if (s == 0) {
       constexpr int CU = (PSWEEP_MAX_N + NT - 1) / NT;
@@ -1706,10 +1786,16 @@ __global__ void __launch_bounds__(NT) k_psweep_ll(const T *__restrict__ G, const
 #pragma unroll
         for (int u = 0; u < CU; ++u) {
           const int c = tid + u * NT;
-          const bool ok = dir == 0 && q < nrhs && c < ny;
-          const double *rp = r + 2 * ((size_t)q * M + (size_t)k * ny + c);
-          ar[q][u] = ok ? rp[0] : 0.0;
-          ai[q][u] = ok ? rp[1] : 0.0;
+          const bool ok = q < nrhs && c < ny;
+          cd v = {0, 0};
+          if (ok && dir == 0) {
+            v = ld_c(r, (size_t)q * M + (size_t)k * ny + c);
+          } else if (ok && ch.from_twist) {
+            const T *yt = y + 2 * (((size_t)q * nx + ch.t) * np + c);
+            v = cmul(ld_c(op, (size_t)cpl_plane * M + (size_t)k * ny + c), cd{(double)yt[0], (double)yt[1]});
+          }
+          ar[q][u] = v.r;
+          ai[q][u] = v.i;
         }
 #pragma unroll
       for (int q = 0; q < RP; ++q)
@@ -1722,7 +1808,7 @@ __global__ void __launch_bounds__(NT) k_psweep_ll(const T *__restrict__ G, const
           }
         }
     } else {
-      const u64 *vin = sy->llbuf + (size_t)(s & 1) * vstride;
+      const u64 *vin = cs->llbuf + (size_t)(s & 1) * vstride;
       const unsigned tag = tag0 + (unsigned)s;
       unsigned spins = 0;
       bool give_up = false;
@@ -1806,13 +1892,13 @@ __global__ void __launch_bounds__(NT) k_psweep_ll(const T *__restrict__ G, const
       else if (my_a < ny) x[2 * ((size_t)my_q * M + (size_t)k * ny + my_a) + my_ri] += (double)out;
     }
     // 5. publish the next block's input entry for this row:
-    //    forward v_{k+1} = r_{k+1} - XM_{k+1} y_k, backward w_{k-1} = XP_{k-1} d_k
+    //    forward v_kn = r_kn - cpl(kn) y_k, backward w_kn = cpl(kn) d_k
     const T other = __shfl_xor(out, 1 << SH);   // the other component of this (row, rhs)
     if (my_store && has_next) {
       const double ore = (double)(my_ri == 0 ? out : other), oim = (double)(my_ri == 0 ? other : out);
       const cd t = cmul(cn, cd{ore, oim});
       const cd v = dir == 0 ? csub(rn, t) : t;
-      u64 *vout = sy->llbuf + (size_t)((s + 1) & 1) * vstride;
+      u64 *vout = cs->llbuf + (size_t)((s + 1) & 1) * vstride;
       publish_tagged(vout + (((size_t)my_q * np + my_a) * 2 + my_ri) * WPC, (T)(my_ri == 0 ? v.r : v.i),
                      tag0 + (unsigned)(s + 1));
     }
@@ -1838,12 +1924,13 @@ __global__ void __launch_bounds__(NT) k_psweep_ll(const T *__restrict__ G, const
 // With PMLs of at least WRAP_MIN_PML cells on both axes the far corners
 // T^-1_{0,m-1} and T^-1_{m-1,0} (waves crossing the whole domain and both
 // x-PMLs) are dropped and the Sigma chain starts at block
-// L = 2 NPML_x + WRAP_EXTRA with Sigma_L = G_L (the part of T^-1_{0,0} that
-// returns from beyond L has crossed the x-PML twice). K is then approximate
-// to about 1e-9 and a FP64 solve takes two refinement steps instead of one,
-// while the chains shrink from 3 (m - 1) products to 2 L. Without such PMLs
-// (periodic axes, where waves along the blocks are never absorbed) the
-// chains run over all blocks and K is exact.
+// L = NPML_x + WRAP_EXTRA, past the x-PML, with Sigma_L = G_L. What the
+// truncation leaves out is the part of T^-1_{0,0} reflected from beyond L,
+// which has crossed the x-PML twice and does not shrink with a longer chain;
+// the refinement takes as many steps with L = NPML_x + 8 as with
+// 2 NPML_x + 16. The chains shrink from 3 (m - 1) products to 2 L. Without
+// such PMLs (periodic axes, where waves along the blocks are never absorbed)
+// the chains run over all blocks and K is exact.
 
 // C = D + sign * A * (rs . B . cs) on n_p x n_p planar matrices, 32 x 32 output
 // tiles per workgroup, MFMA complex products (rs / cs: optional complex row /
@@ -1946,7 +2033,7 @@ __device__ inline cd stretch_d(const fdfd_grid &g, int i, int j) {
 }
 
 // K = I + V^T T^-1 U C as a row-major complex FP64 matrix of order 2n
-constexpr int WRAP_MIN_PML = 4, WRAP_EXTRA = 16;
+constexpr int WRAP_MIN_PML = 4, WRAP_EXTRA = 8;
 
 // Y = NULL: the far corners are taken as zero
 template <typename T>
@@ -2403,12 +2490,26 @@ size_t solve_work_bytes(const fdfd_grid *g, int precision, int nrhs) {
 // Exact wrap correction (full chains, K of order 2n) or the truncated one;
 // the factorization and the solve take the same decision.
 inline bool wrap_exact(const fdfd_grid *g) {
-  return g->npml_x < WRAP_MIN_PML || g->npml_y < WRAP_MIN_PML || 2 * g->npml_x + WRAP_EXTRA >= g->nx - 1;
+  return g->npml_x < WRAP_MIN_PML || g->npml_y < WRAP_MIN_PML || g->npml_x + WRAP_EXTRA >= g->nx - 1;
+}
+// The twist block c of the factorization. Twisted (c < m - 1): a top chain
+// eliminates blocks 0..c-1 downwards (G_k = S_k^-1), a bottom chain blocks
+// m-1..c+1 upwards (G_k = S'_k^-1, S'_k = T_k - XP_k G_{k+1} XM_{k+1}), both
+// in the same step launches, and block c takes both:
+// G_c = (T_c - XM_c G_{c-1} XP_{c-1} - XP_c G_{c+1} XM_{c+1})^-1. The serial
+// chain of a factorization and of a sweep is then about half as long. It
+// needs the truncated wrap with room for both corner chains (L < c and
+// L <= m - 2 - c); otherwise one chain over all blocks (c = m - 1).
+inline int twist_block(const fdfd_grid *g) {
+  const int m = g->nx;
+  if (!g_twist.load() || wrap_exact(g)) return m - 1;
+  const int c = (m + 1) / 2, L = g->npml_x + WRAP_EXTRA;
+  return (L < c && L <= m - 2 - c) ? c : m - 1;
 }
 // The factor buffer: the block inverses G_k ([m][2][np][np] of T), then the
 // wrap correction: K^-1 (row-major complex FP64, order 2n) when exact, else
-// S^-1 (order n, planar and padded to np when tiled) followed by a copy of
-// Sigma_0 (planar T, np x np).
+// S^-1 (order n, planar and padded to np when tiled) followed by copies of
+// Sigma_0 and of T^-1_{m-1,m-1} (planar T, np x np each).
 size_t factor_blocks_bytes(const fdfd_grid *g, int precision) {
   const size_t np = pad_n(g->ny);
   return align_up((size_t)g->nx * 2 * np * np * elem_size(precision), 256);
@@ -2428,13 +2529,19 @@ inline bool wrap_sinv_tiled(const fdfd_grid *g, int precision) {
 size_t factor_total_bytes(const fdfd_grid *g, int precision) {
   const size_t n = g->ny, np = pad_n(g->ny);
   return factor_blocks_bytes(g, precision) +
-         std::max(64 * n * n, wrap_sinv_bytes(g) + 2 * np * np * elem_size(precision));
+         std::max(64 * n * n, wrap_sinv_bytes(g) + 4 * np * np * elem_size(precision));
 }
 inline double *wrap_kinv(const fdfd_grid *g, int precision, const void *factor) {
   return (double *)((char *)factor + factor_blocks_bytes(g, precision));
 }
 inline void *wrap_sigma(const fdfd_grid *g, int precision, const void *factor) {
   return (char *)factor + factor_blocks_bytes(g, precision) + wrap_sinv_bytes(g);
+}
+// T^-1_{m-1,m-1} of the truncated wrap: G_{m-1} of a single chain, or the
+// mirrored chain's Sigma'_{m-1} of a twisted factorization
+inline void *wrap_sigma_last(const fdfd_grid *g, int precision, const void *factor) {
+  const size_t np = pad_n(g->ny);
+  return (char *)wrap_sigma(g, precision, factor) + 2 * np * np * elem_size(precision);
 }
 
 template <typename T, bool PIV>
@@ -2477,28 +2584,59 @@ int factor_persistent(const fdfd_grid *g, const double *op, void *factor, void *
   return FDFD_OK;
 }
 
+// The chain jobs of one factorization: the whole chain, or the top and
+// bottom chains of a twisted one. The bottom chain works in the second half
+// of W (room for the persistent kernel's two exchange parities) and keeps
+// its pivot inverses in the persistent kernel's pivot area, both unused by
+// the launch-per-step path otherwise.
+// J has room for two jobs; only J[0] is written for a single chain.
+template <typename T>
+int chain_jobs(const fdfd_grid *g, const double *op, void *factor, void *work, bool sym, FJob<T> *J) {
+  const int n = g->ny, np = pad_n(n), nt = np / TB, m = g->nx, c = twist_block(g);
+  FactorWork fw = factor_work_layout(g, sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64, work);
+  memset(J, 0, sizeof(FJob<T>));
+  J[0].op = op; J[0].G = (T *)factor; J[0].W = (T *)fw.W; J[0].Pinv = (T *)fw.Pinv; J[0].status = fw.status;
+  J[0].dsym = sym ? (T *)fw.xcol : nullptr;
+  J[0].m = m; J[0].n = n; J[0].np = np; J[0].nt = nt;
+  J[0].k = -1; J[0].p = 0; J[0].dir = 1; J[0].c = c;
+  if (c == m - 1) return 1;
+  J[1] = J[0];
+  J[1].W = (T *)fw.W + 2 * (size_t)np * np;
+  J[1].Pinv = (T *)fw.pinv_all;
+  J[1].dir = -1;
+  return 2;
+}
+// Steps of a chain job after its initial pivot: the top chain's
+// (c + 1) nt; the bottom chain's (m - 1 - c) nt end earlier.
+template <typename T>
+inline int chain_steps(const FJob<T> &J) {
+  return (J.dir > 0 ? J.c + 1 : J.m - 1 - J.c) * J.nt;
+}
+template <typename T>
+inline void chain_at(FJob<T> &J, int st) {
+  J.p = st % J.nt;
+  J.k = J.dir > 0 ? st / J.nt : J.m - 1 - st / J.nt;
+}
+
 template <typename T, bool PIV>
 int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, hipStream_t s) {
-  const int n = g->ny, np = pad_n(n), nt = np / TB, m = g->nx;
   FactorWork fw = factor_work_layout(g, sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64, work);
-  (void)PIV;
   HIP_TRY(hipMemsetAsync(fw.status, 0, sizeof(int), s));
   FBatch<T> b;
   memset(&b, 0, sizeof b);
-  b.njobs = 1;
-  FJob<T> &J = b.job[0];
-  J.op = op; J.G = (T *)factor; J.W = (T *)fw.W; J.Pinv = (T *)fw.Pinv; J.status = fw.status;
-  J.dsym = g_step_sym.load() ? (T *)fw.xcol : nullptr;
-  J.m = m; J.n = n; J.np = np; J.nt = nt;
-  J.k = -1; J.p = 0;
+  const int nch = chain_jobs<T>(g, op, factor, work, g_step_sym.load() != 0, b.job);
+  b.njobs = nch;
   int rc = launch_step<T, PIV>(b, s);
   if (rc) return rc;
-  for (int k = 0; k < m; ++k) {
-    for (int p = 0; p < nt; ++p) {
-      J.k = k; J.p = p;
-      rc = launch_step<T, PIV>(b, s);
-      if (rc) return rc;
-    }
+  const int steps = chain_steps(b.job[0]);
+  for (int st = 0; st < steps; ++st) {
+    b.njobs = 0;
+    for (int i = 0; i < nch; ++i)
+      if (st < chain_steps(b.job[i])) {
+        chain_at(b.job[i], st);
+        b.njobs = i + 1;
+      }
+    if ((rc = launch_step<T, PIV>(b, s))) return rc;
   }
   return FDFD_OK;
 }
@@ -2508,48 +2646,90 @@ int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, 
 template <typename T, bool PIV>
 int factor_batch_impl(const fdfd_grid *g, const double *const *ops, void *const *factors, void *const *works,
                       int njobs, hipStream_t s) {
-  const int n = g->ny, np = pad_n(n), nt = np / TB, m = g->nx;
   FBatch<T> b;
   memset(&b, 0, sizeof b);
-  b.njobs = njobs;
+  FJob<T> top[MAX_BATCH], bot[MAX_BATCH];
+  int nch = 1;
   for (int j = 0; j < njobs; ++j) {
     FactorWork fw = factor_work_layout(g, sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64, works[j]);
     HIP_TRY(hipMemsetAsync(fw.status, 0, sizeof(int), s));
-    FJob<T> &J = b.job[j];
-    J.op = ops[j]; J.G = (T *)factors[j]; J.W = (T *)fw.W; J.Pinv = (T *)fw.Pinv; J.status = fw.status;
-    J.dsym = g_step_sym.load() ? (T *)fw.xcol : nullptr;
-    J.m = m; J.n = n; J.np = np; J.nt = nt;
-    J.k = -1; J.p = 0;
+    FJob<T> pair[2];
+    nch = chain_jobs<T>(g, ops[j], factors[j], works[j], g_step_sym.load() != 0, pair);
+    top[j] = pair[0];
+    bot[j] = pair[1];
   }
+  // every chain's initial pivot, then the steps: the top chains (and, while
+  // they last, the bottom chains) of all operators in each launch
+  auto fill = [&](int st) {
+    b.njobs = 0;
+    for (int j = 0; j < njobs; ++j) {
+      if (st >= 0) chain_at(top[j], st);
+      b.job[b.njobs++] = top[j];
+    }
+    if (nch == 2 && (st < 0 || st < chain_steps(bot[0])))
+      for (int j = 0; j < njobs; ++j) {
+        if (st >= 0) chain_at(bot[j], st);
+        b.job[b.njobs++] = bot[j];
+      }
+  };
+  fill(-1);
   int rc = launch_step<T, PIV>(b, s);
   if (rc) return rc;
-  for (int k = 0; k < m; ++k) {
-    for (int p = 0; p < nt; ++p) {
-      for (int j = 0; j < njobs; ++j) { b.job[j].k = k; b.job[j].p = p; }
-      if ((rc = launch_step<T, PIV>(b, s))) return rc;
-    }
+  const int steps = chain_steps(top[0]);
+  for (int st = 0; st < steps; ++st) {
+    fill(st);
+    if ((rc = launch_step<T, PIV>(b, s))) return rc;
   }
   return FDFD_OK;
 }
 
 template <typename T>
-int sweep_blockwise(const fdfd_grid *g, const double *op, const void *factor, const double *r,
-                    double *x, int nrhs, void *y, hipStream_t s) {
-  const int nx = g->nx, ny = g->ny, np = pad_n(ny);
-  const dim3 grid((np + 3) / 4);
-  const size_t lds = (size_t)2 * MAXR * np * sizeof(T);
+void sweep_block_attr() {
   static std::once_flag lds_attr;
   std::call_once(lds_attr, [] {
     const int most = (int)(2 * MAXR * pad_n(MAX_BLOCK_N) * sizeof(T));
     (void)hipFuncSetAttribute((const void *)k_sweep_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, most);
     (void)hipFuncSetAttribute((const void *)k_sweep_bwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, most);
   });
-  for (int k = 0; k < nx; ++k)
-    hipLaunchKernelGGL(k_sweep_fwd<T>, grid, dim3(NT), lds, s, (const T *)factor, op, r, (T *)y, nx, ny, np, k,
-                       nrhs);
-  for (int k = nx - 1; k >= 0; --k)
-    hipLaunchKernelGGL(k_sweep_bwd<T>, grid, dim3(NT), lds, s, (const T *)factor, op, (T *)y, x, nx, ny, np, k,
-                       nrhs);
+}
+
+// y_c and x_c += y_c of the twist block, between the forward chains and the
+// backward chains of a twisted sweep
+template <typename T>
+void launch_twist_block(const fdfd_grid *g, const T *G, const double *op, const double *r, T *y, double *x,
+                        int nrhs, int c, hipStream_t s) {
+  const int np = pad_n(g->ny);
+  sweep_block_attr<T>();
+  hipLaunchKernelGGL(k_sweep_fwd<T>, dim3((np + 3) / 4), dim3(NT), (size_t)2 * MAXR * np * sizeof(T), s, G, op, r,
+                     y, x, g->nx, g->ny, np, c, nrhs, c - 1, (int)FDFD_OP_XM, c + 1, (int)FDFD_OP_XP);
+}
+
+template <typename T>
+int sweep_blockwise(const fdfd_grid *g, const double *op, const void *factor, const double *r,
+                    double *x, int nrhs, void *y, hipStream_t s) {
+  const int nx = g->nx, ny = g->ny, np = pad_n(ny), c = twist_block(g);
+  const dim3 grid((np + 3) / 4);
+  const size_t lds = (size_t)2 * MAXR * np * sizeof(T);
+  const T *G = (const T *)factor;
+  T *yt = (T *)y;
+  sweep_block_attr<T>();
+  auto fwd = [&](int k, int nb, int pl) {
+    hipLaunchKernelGGL(k_sweep_fwd<T>, grid, dim3(NT), lds, s, G, op, r, yt, (double *)nullptr, nx, ny, np, k,
+                       nrhs, nb, pl, -1, 0);
+  };
+  auto bwd = [&](int k, int nb, int pl) {
+    hipLaunchKernelGGL(k_sweep_bwd<T>, grid, dim3(NT), lds, s, G, op, yt, x, nx, ny, np, k, nrhs, nb, pl);
+  };
+  if (c == nx - 1) {
+    for (int k = 0; k < nx; ++k) fwd(k, k > 0 ? k - 1 : -1, FDFD_OP_XM);
+    for (int k = nx - 1; k >= 0; --k) bwd(k, k + 1 < nx ? k + 1 : -1, FDFD_OP_XP);
+  } else {
+    for (int k = 0; k < c; ++k) fwd(k, k > 0 ? k - 1 : -1, FDFD_OP_XM);
+    for (int k = nx - 1; k > c; --k) fwd(k, k + 1 < nx ? k + 1 : -1, FDFD_OP_XP);
+    launch_twist_block<T>(g, G, op, r, yt, x, nrhs, c, s);
+    for (int k = c - 1; k >= 0; --k) bwd(k, k + 1, FDFD_OP_XP);
+    for (int k = c + 1; k < nx; ++k) bwd(k, k - 1, FDFD_OP_XM);
+  }
   HIP_TRY(hipGetLastError());
   return FDFD_OK;
 }
@@ -2570,7 +2750,7 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
   const T *Glast = G + (size_t)(m - 1) * 2 * np2;
   // exact chains over all blocks, or the truncated Sigma chain without Z
   const bool exact = wrap_exact(g);
-  const int L = exact ? m - 1 : 2 * g->npml_x + WRAP_EXTRA;
+  const int L = exact ? m - 1 : g->npml_x + WRAP_EXTRA;
   const T *sig_prev = G + (size_t)L * 2 * np2, *z_prev = exact ? Glast : nullptr;
   int zi = 0;
   for (int k = L - 1; k >= 0; --k) {
@@ -2605,11 +2785,38 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
     hipLaunchKernelGGL(k_wrap_capacitance<T>, dim3((N + NT - 1) / NT, N), dim3(NT), 0, s, *g, op, Glast, sig_prev,
                        z_prev, np, K);
   } else {
-    // S = I - Sigma_0 XM_0 G_{m-1} XP_{m-1} (order n), and a copy of Sigma_0
+    // copies of Sigma_0 and of T^-1_{m-1,m-1}: G_{m-1}, or for a twisted
+    // factorization the mirrored truncated chain over the bottom blocks,
+    // Sigma'_{m-1-L} = G_{m-1-L}, Sigma'_k = G_k + G_k XM_k Sigma'_{k-1} XP_{k-1} G_k
+    T *sig0 = (T *)wrap_sigma(g, prec, factor), *sigl = (T *)wrap_sigma_last(g, prec, factor);
+    HIP_TRY(hipMemcpyAsync(sig0, sig_prev, 2 * np2 * sizeof(T), hipMemcpyDeviceToDevice, s));
+    if (twist_block(g) < m - 1) {
+      const T *sp = G + (size_t)(m - 1 - L) * 2 * np2;
+      for (int k = m - L; k <= m - 1; ++k) {
+        const T *Gk = G + (size_t)k * 2 * np2;
+        GemmBatch<T> b1;
+        memset(&b1, 0, sizeof b1);
+        b1.njobs = 1;
+        b1.job[0] = GemmJob<T>{Gk, sp, nullptr, Q, op + 2 * (FDFD_OP_XM * M + (size_t)k * n),
+                               op + 2 * (FDFD_OP_XP * M + (size_t)(k - 1) * n), T(1)};
+        hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(GT), 0, s, b1, n, np);
+        GemmBatch<T> b2;
+        memset(&b2, 0, sizeof b2);
+        b2.njobs = 1;
+        b2.job[0] = GemmJob<T>{Q, Gk, Gk, Sig, nullptr, nullptr, T(1)};
+        hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(GT), 0, s, b2, n, np);
+        sp = Sig;
+      }
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(sigl, Sig, 2 * np2 * sizeof(T), hipMemcpyDeviceToDevice, s));
+    } else {
+      HIP_TRY(hipMemcpyAsync(sigl, Glast, 2 * np2 * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
+    // S = I - Sigma_0 XM_0 T^-1_{m-1,m-1} XP_{m-1} (order n)
     GemmBatch<T> b3;
     memset(&b3, 0, sizeof b3);
     b3.njobs = 1;
-    b3.job[0] = GemmJob<T>{sig_prev, Glast, nullptr, Q, op + 2 * (FDFD_OP_XM * M),
+    b3.job[0] = GemmJob<T>{sig0, sigl, nullptr, Q, op + 2 * (FDFD_OP_XM * M),
                            op + 2 * (FDFD_OP_XP * M + (size_t)(m - 1) * n), T(-1)};
     hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(GT), 0, s, b3, n, np);
     const bool tiled = wrap_sinv_tiled(g, prec);
@@ -2621,10 +2828,9 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
       HIP_TRY(hipMemsetAsync(K, 0, wrap_sinv_bytes(g), s));
       hipLaunchKernelGGL(k_wrap_schur<T>, dim3((n + NT - 1) / NT, n), dim3(NT), 0, s, (const T *)Q, np, n, K, 0);
     }
-    HIP_TRY(hipMemcpyAsync(wrap_sigma(g, prec, factor), sig_prev, 2 * np2 * sizeof(T), hipMemcpyDeviceToDevice, s));
     // the rest of the tail (sized for the exact form) is cleared, so a factor
     // buffer is a deterministic function of the operator
-    const size_t used = factor_blocks_bytes(g, prec) + wrap_sinv_bytes(g) + 2 * np2 * sizeof(T);
+    const size_t used = factor_blocks_bytes(g, prec) + wrap_sinv_bytes(g) + 4 * np2 * sizeof(T);
     const size_t total = factor_total_bytes(g, prec);
     if (total > used) HIP_TRY(hipMemsetAsync((char *)factor + used, 0, total - used, s));
     if (tiled) {
@@ -2638,7 +2844,7 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
       J.op = op; J.G = K; J.W = (double *)fw.W; J.Pinv = (double *)fw.pinv_all; J.status = fw.status;
       J.dense = fw.kscratch; J.dsym = nullptr;
       J.m = 1; J.n = n; J.np = np; J.nt = nt;
-      J.k = -1; J.p = 0;
+      J.k = -1; J.p = 0; J.dir = 1; J.c = 0;
       int rc = pivoted ? launch_step<double, true>(b, s) : launch_step<double, false>(b, s);
       for (int p = 0; p < nt && !rc; ++p) {
         J.k = 0; J.p = p;
@@ -2675,9 +2881,9 @@ int wrap_lag_forcing(const fdfd_grid *g, const double *op, const void *factor, c
                      double *r, hipStream_t s) {
   const int prec = sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64;
   const int m = g->nx, n = g->ny, np = pad_n(n);
-  const size_t M = (size_t)m * n, np2 = (size_t)np * np, nv = 2 * (size_t)n;
+  const size_t M = (size_t)m * n, nv = 2 * (size_t)n;
   const T *sigma = (const T *)wrap_sigma(g, prec, factor);
-  const T *Glast = (const T *)factor + (size_t)(m - 1) * 2 * np2;
+  const T *Glast = (const T *)wrap_sigma_last(g, prec, factor);   // T^-1_{m-1,m-1}
   const double *sinv = wrap_kinv(g, prec, factor);
   const double *XM0 = op + 2 * (FDFD_OP_XM * M), *XPl = op + 2 * (FDFD_OP_XP * M + (size_t)(m - 1) * n);
   double *lag = sw.f;
@@ -2706,7 +2912,8 @@ int wrap_lag_forcing(const fdfd_grid *g, const double *op, const void *factor, c
 
 template <typename T, int RP>
 void launch_psweep(int W, size_t lds, hipStream_t s, const T *G, const double *op, const double *r, T *y,
-                   double *x, int nx, int ny, int np, int nr, int dir, SweepSync *sy, unsigned tag0) {
+                   double *x, int nx, int ny, int np, int nr, int dir, const SweepChains &chs, SweepSync *sy,
+                   unsigned tag0) {
   static std::once_flag lds_attr;
   std::call_once(lds_attr, [] {
     if constexpr (RP <= LL_MAXRHS)
@@ -2717,11 +2924,31 @@ void launch_psweep(int W, size_t lds, hipStream_t s, const T *G, const double *o
                                 (int)(2 * RP * PSWEEP_MAX_N * sizeof(T)));
   });
   if constexpr (RP <= LL_MAXRHS)
-    hipLaunchKernelGGL((k_psweep_ll<T, RP>), dim3(W), dim3(NT), lds, s, G, op, r, y, x, nx, ny, np, nr, dir, sy,
-                       g_spin_cap.load(), tag0);
+    hipLaunchKernelGGL((k_psweep_ll<T, RP>), dim3(W * chs.nch), dim3(NT), lds, s, G, op, r, y, x, nx, ny, np, nr,
+                       dir, chs, sy, g_spin_cap.load(), tag0);
   else
-    hipLaunchKernelGGL((k_psweep<T, RP>), dim3(W), dim3(NT), lds, s, G, op, r, y, x, nx, ny, np, nr, dir, sy,
-                       g_spin_cap.load());
+    hipLaunchKernelGGL((k_psweep<T, RP>), dim3(W * chs.nch), dim3(NT), lds, s, G, op, r, y, x, nx, ny, np, nr,
+                       dir, chs, sy, g_spin_cap.load());
+}
+
+// The chains of a persistent sweep: dir 0 forward, 1 backward.
+SweepChains sweep_chains(const fdfd_grid *g, int dir) {
+  const int m = g->nx, c = twist_block(g);
+  SweepChains chs;
+  memset(&chs, 0, sizeof chs);
+  if (c == m - 1) {
+    chs.nch = 1;
+    chs.ch[0] = dir == 0 ? SweepChain{0, 1, m, FDFD_OP_XM, 0, 0} : SweepChain{m - 1, -1, m, FDFD_OP_XP, 0, 0};
+  } else if (dir == 0) {
+    chs.nch = 2;
+    chs.ch[0] = SweepChain{0, 1, c, FDFD_OP_XM, 0, 0};
+    chs.ch[1] = SweepChain{m - 1, -1, m - 1 - c, FDFD_OP_XP, 0, 0};
+  } else {
+    chs.nch = 2;
+    chs.ch[0] = SweepChain{c - 1, -1, c, FDFD_OP_XP, 1, c};
+    chs.ch[1] = SweepChain{c + 1, 1, m - 1 - c, FDFD_OP_XM, 1, c};
+  }
+  return chs;
 }
 
 // x += F^-1 r with the persistent sweep kernels (RHS in passes of
@@ -2731,9 +2958,10 @@ int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const d
                double *x, int nrhs, SolveWork &sw, hipStream_t s, bool blockwise) {
   // blocks beyond the persistent kernel's register tiles: one launch per block
   if (blockwise || pad_n(g->ny) > PSWEEP_MAX_N) return sweep_blockwise<T>(g, op, factor, r, x, nrhs, sw.y, s);
-  const int nx = g->nx, ny = g->ny, np = pad_n(ny);
+  const int nx = g->nx, ny = g->ny, np = pad_n(ny), c = twist_block(g);
   const int W = (np + 4 * SRW - 1) / (4 * SRW);
   const size_t M = (size_t)nx * ny;
+  const SweepChains chs[2] = {sweep_chains(g, 0), sweep_chains(g, 1)};
   for (int q0 = 0; q0 < nrhs; q0 += sweep_rhs<T>()) {
     const int nr = std::min(sweep_rhs<T>(), nrhs - q0);
     const int rp = nr <= 1 ? 1 : nr <= 2 ? 2 : nr <= 4 ? 4 : 8;
@@ -2743,10 +2971,11 @@ int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const d
     // across the sweeps of one solve attempt. Flag sweep: the step flags
     // start at zero. Tagged-word sweep: each launch's tags start past every
     // tag written before in this workspace since it was cleared.
-    if (rp > LL_MAXRHS) HIP_TRY(hipMemsetAsync(sw.sync->done, 0, sizeof sw.sync->done, s));
+    if (rp > LL_MAXRHS)
+      for (int i = 0; i < chs[0].nch; ++i) HIP_TRY(hipMemsetAsync(sw.sync->ch[i].done, 0, sizeof sw.sync->ch[i].done, s));
     for (int dir = 0; dir < 2; ++dir) {
       if (sw.tag_next > 0xffffffffu - (unsigned)nx - 2) {
-        HIP_TRY(hipMemsetAsync(sw.sync->llbuf, 0, sizeof sw.sync->llbuf, s));
+        for (int i = 0; i < 2; ++i) HIP_TRY(hipMemsetAsync(sw.sync->ch[i].llbuf, 0, sizeof sw.sync->ch[i].llbuf, s));
         sw.tag_next = 1;
       }
       const unsigned tag0 = sw.tag_next;
@@ -2754,15 +2983,19 @@ int sweep_impl(const fdfd_grid *g, const double *op, const void *factor, const d
       const double *rq = dir == 0 ? r + 2 * M * q0 : nullptr;
       double *xq = dir == 1 ? x + 2 * M * q0 : nullptr;
       const T *G = (const T *)factor;
+      const SweepChains &ch = chs[dir];
       switch (rp) {
-        case 1: launch_psweep<T, 1>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync, tag0); break;
-        case 2: launch_psweep<T, 2>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync, tag0); break;
-        case 4: launch_psweep<T, 4>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync, tag0); break;
+        case 1: launch_psweep<T, 1>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, ch, sw.sync, tag0); break;
+        case 2: launch_psweep<T, 2>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, ch, sw.sync, tag0); break;
+        case 4: launch_psweep<T, 4>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, ch, sw.sync, tag0); break;
         default:
           if constexpr (sweep_rhs<T>() >= 8)
-            launch_psweep<T, 8>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, sw.sync, tag0);
+            launch_psweep<T, 8>(W, lds, s, G, op, rq, yq, xq, nx, ny, np, nr, dir, ch, sw.sync, tag0);
           break;
       }
+      // twisted: the twist block between the two passes
+      if (dir == 0 && c < nx - 1)
+        launch_twist_block<T>(g, G, op, r + 2 * M * q0, yq, x + 2 * M * q0, nr, c, s);
     }
   }
   HIP_TRY(hipGetLastError());
@@ -2794,7 +3027,8 @@ struct EngineJob {
   bool pivot;                  // partial pivoting inside pivot tiles
   bool sym = true;             // symmetric steps (upper tiles only)
   int m, n, np, nt;
-  int k = -1, p = 0;          // next step; k = -1: initial pivot
+  int step = -1;              // next step of the chains; -1: the initial pivots
+  int nch = 1, steps = 0;     // chains (2: twisted) and the top chain's steps
   bool launched_all = false;  // every step is on the stream
   hipEvent_t done = nullptr;
   int rc = FDFD_OK;
@@ -2819,12 +3053,21 @@ struct fdfd_engine {
 
 namespace {
 
+// The chain jobs of an engine job at its next step into J (room for e.nch
+// jobs); returns how many take part.
 template <typename T>
-void fill_job(FJob<T> &J, const EngineJob &e) {
-  FactorWork fw = factor_work_layout(&e.g, e.precision, e.work);
-  J.op = e.op; J.G = (T *)e.factor; J.W = (T *)fw.W; J.Pinv = (T *)fw.Pinv; J.status = fw.status;
-  J.dsym = e.sym ? (T *)fw.xcol : nullptr;
-  J.m = e.m; J.n = e.n; J.np = e.np; J.nt = e.nt; J.k = e.k; J.p = e.p;
+int fill_jobs(FJob<T> *J, const EngineJob &e) {
+  FJob<T> pair[2];
+  chain_jobs<T>(&e.g, e.op, e.factor, e.work, e.sym, pair);
+  int n = 0;
+  for (int i = 0; i < e.nch; ++i) {
+    if (e.step >= 0) {
+      if (e.step >= chain_steps(pair[i])) break;   // the bottom chain ends first
+      chain_at(pair[i], e.step);
+    }
+    J[n++] = pair[i];
+  }
+  return n;
 }
 
 // One batched launch over up to MAXJ jobs of precision T; advances their
@@ -2837,17 +3080,14 @@ template <typename T, bool PIV>
 int engine_launch(fdfd_engine *E, std::vector<std::shared_ptr<EngineJob>> &sel) {
   FBatch<T> b;
   memset(&b, 0, sizeof b);
-  b.njobs = (int)sel.size();
-  for (int j = 0; j < b.njobs; ++j) fill_job<T>(b.job[j], *sel[j]);
+  for (auto &e : sel) b.njobs += fill_jobs<T>(b.job + b.njobs, *e);
   const int wg = batch_layout(b);
   hipLaunchKernelGGL((k_gj_step<T, PIV>), dim3(wg), dim3(NT), 0, E->stream, b);
   const hipError_t err = hipGetLastError();
   std::vector<std::pair<EngineJob *, int>> finished;  // job, rc
   for (auto &e : sel) {
     if (err != hipSuccess) { finished.emplace_back(e.get(), FDFD_ERR_HIP); continue; }
-    if (e->k < 0) { e->k = 0; e->p = 0; continue; }
-    if (++e->p == e->nt) { e->p = 0; ++e->k; }
-    if (e->k == e->m)
+    if (++e->step == e->steps)
       finished.emplace_back(e.get(), hipEventRecord(e->done, E->stream) == hipSuccess ? FDFD_OK : FDFD_ERR_HIP);
   }
   std::lock_guard<std::mutex> lk(E->mu);
@@ -2885,8 +3125,12 @@ void engine_loop(fdfd_engine *E) {
       for (int t = 1; t <= 4; ++t)
         if (has[(E->last_prec + t) % 4]) { prec = (E->last_prec + t) % 4; break; }
       E->last_prec = prec;
+      int slots = 0;   // chains in the launch
       for (auto &e : E->active)
-        if (e->precision * 2 + (e->pivot ? 1 : 0) == prec && (int)sel.size() < MAXJ) sel.push_back(e);
+        if (e->precision * 2 + (e->pivot ? 1 : 0) == prec && slots + e->nch <= MAXJ) {
+          sel.push_back(e);
+          slots += e->nch;
+        }
     }
     switch (prec) {
       case 0: engine_launch<double, false>(E, sel); break;
@@ -2902,7 +3146,7 @@ void engine_loop(fdfd_engine *E) {
         else ++it;
       }
       // round robin: jobs that just ran go to the back
-      if (!any && E->active.size() > (size_t)MAXJ)
+      if (!any && E->active.size() > sel.size())
         std::rotate(E->active.begin(), E->active.begin() + std::min(sel.size(), E->active.size()),
                     E->active.end());
       if (any) E->cv_done.notify_all();
@@ -3041,7 +3285,8 @@ int fdfd_factor(const fdfd_grid *g, const double *op, void *factor, int precisio
   const bool piv = precision & FDFD_FACTOR_PIVOT;
   const bool fp64 = base_prec(precision) == FDFD_PREC_FP64;
   bool ran = false;
-  if (!(precision & FDFD_FACTOR_STEPWISE)) {
+  // the persistent kernel factors one chain; a twisted factorization is stepwise
+  if (!(precision & FDFD_FACTOR_STEPWISE) && twist_block(g) == g->nx - 1) {
     int rc = fp64 ? (piv ? factor_persistent<double, true>(g, op, factor, work, stream, &ran)
                          : factor_persistent<double, false>(g, op, factor, work, stream, &ran))
                   : (piv ? factor_persistent<float, true>(g, op, factor, work, stream, &ran)
@@ -3063,8 +3308,8 @@ int fdfd_factor_batch(const fdfd_grid *g, const double *const *ops, void *const 
                       int precision, void *const *works, hipStream_t stream) {
   if (int rc = check_grid(g)) return rc;
   if (int rc = check_prec(precision)) return rc;
-  if (!ops || !factors || !works || njobs < 1 || njobs > MAXJ)
-    return fail(FDFD_ERR_ARG, "bad arguments to fdfd_factor_batch (1 <= njobs <= %d)", MAXJ);
+  if (!ops || !factors || !works || njobs < 1 || njobs > MAX_BATCH)
+    return fail(FDFD_ERR_ARG, "bad arguments to fdfd_factor_batch (1 <= njobs <= %d)", MAX_BATCH);
   for (int j = 0; j < njobs; ++j)
     if (!ops[j] || !factors[j] || !works[j]) return fail(FDFD_ERR_ARG, "fdfd_factor_batch: job %d has a NULL pointer", j);
   const bool piv = precision & FDFD_FACTOR_PIVOT;
@@ -3404,6 +3649,11 @@ int fdfd_engine_submit(fdfd_engine *E, const fdfd_grid *g, const double *op, voi
   e->precision = base_prec(precision); e->pivot = precision & FDFD_FACTOR_PIVOT;
   e->m = g->nx; e->n = g->ny; e->np = pad_n(g->ny); e->nt = e->np / TB;
   e->sym = g_step_sym.load() != 0;
+  {
+    const int c = twist_block(g);
+    e->nch = c < e->m - 1 ? 2 : 1;
+    e->steps = (c + 1) * e->nt;
+  }
   HIP_TRY(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
   {
     std::lock_guard<std::mutex> lk(E->mu);
@@ -3449,6 +3699,13 @@ void fdfd_set_spin_cap(unsigned polls) { g_spin_cap.store(polls ? polls : SPIN_C
 int fdfd_set_step_symmetry(int on) {
   const int prev = g_step_sym.exchange(on ? 1 : 0);
   return prev;
+}
+
+int fdfd_set_twist(int on) { return g_twist.exchange(on ? 1 : 0); }
+
+int fdfd_twist_block(const fdfd_grid *g) {
+  if (check_grid(g)) return -1;
+  return twist_block(g);
 }
 
 int fdfd_selftest_mfma(int precision, const double *a, const double *b, double *d, hipStream_t stream) {

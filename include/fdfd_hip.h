@@ -264,6 +264,22 @@ void fdfd_set_spin_cap(unsigned polls);
  * previous setting. No reference counterpart. */
 int fdfd_set_step_symmetry(int on);
 
+/* Twisted factorization (the default where it applies, no reference
+ * counterpart): with the truncated x-wrap correction and room for both
+ * corner chains, a top chain eliminates blocks 0..c-1 downwards and a bottom
+ * chain blocks m-1..c+1 upwards, in the same step launches, and block c
+ * takes both (c = fdfd_twist_block(g)); the sweeps run both chains at once
+ * and solve block c between their passes. The factor's blocks, and the
+ * sweeps that read them, follow that order. Off: one chain over all blocks
+ * (the persistent factorization's order). Applies to factorizations and
+ * solves started afterwards; a factor must be solved with the setting it
+ * was built with. Returns the previous setting. */
+int fdfd_set_twist(int on);
+
+/* The twist block c for g under the current setting (m - 1: one chain);
+ * -1 for an invalid grid. */
+int fdfd_twist_block(const fdfd_grid *g);
+
 /* Check of the MFMA operand layouts the factor kernels rely on (no reference
  * counterpart): D = A (16x4) * B (4x16) with the 16x16x4 f64 (precision 0)
  * or f32 (precision 1) instruction; a, b, d are row-major device arrays of

@@ -264,10 +264,13 @@ def test_pivoted_tile_factor_matches_oracle(precision):
 
 @pytest.mark.parametrize("precision,n", [("c64", 200), ("fp64", 96), ("c64", 128)])
 def test_persistent_and_stepwise_factors_identical(precision, n):
-    """The single-launch factorization equals the launch-per-step one bit for bit."""
+    """The single-launch factorization equals the launch-per-step one bit for
+    bit (one chain over all blocks, the persistent kernel's order)."""
     p = C.c3_request(77, n=n)
     ops = []
-    prev = L.lib().fdfd_set_step_symmetry(0)   # full steps: the persistent kernel's arithmetic
+    lib = L.lib()
+    prev = lib.fdfd_set_step_symmetry(0)   # full steps: the persistent kernel's arithmetic
+    prev_tw = lib.fdfd_set_twist(0)
     try:
         for stepwise in (False, True):
             op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, DEV)
@@ -276,8 +279,57 @@ def test_persistent_and_stepwise_factors_identical(precision, n):
             ops.append(op)
         torch.cuda.synchronize()
     finally:
-        L.lib().fdfd_set_step_symmetry(prev)
+        lib.fdfd_set_step_symmetry(prev)
+        lib.fdfd_set_twist(prev_tw)
     assert torch.equal(ops[0].factors[precision][0], ops[1].factors[precision][0])
+
+
+@pytest.mark.parametrize("precision", ["c64", "fp64"])
+def test_twisted_factor_against_one_chain_and_oracle(precision):
+    """The twisted factorization (top chain 0..c-1, bottom chain m-1..c+1,
+    both in the same step launches, then block c) keeps the top blocks of the
+    one-chain factorization bit for bit; its last block is T_{m-1}^-1 (the
+    bottom chain starts there); the twisted sweeps then solve to the oracle's
+    field, as the one-chain factor and sweeps do."""
+    lib = L.lib()
+    p = C.c3_request(21, n=160)
+    m, n = p.shape
+    npad = ((n + 31) // 32) * 32
+    dt = torch.float32 if precision == "c64" else torch.float64
+    rng = np.random.default_rng(21)
+    b = rng.standard_normal((2,) + p.shape) + 1j * rng.standard_normal((2,) + p.shape)
+    sim = OracleSimulation(p.omega, p.eps, p.dl, p.npml, p.pol, p.L0)
+    blocks, xs = {}, {}
+    for twist in (1, 0):
+        prev = lib.fdfd_set_twist(twist)
+        try:
+            op = DeviceOperator(p.shape, p.omega, p.dl, p.npml, p.pol, p.L0, DEV)
+            assert not op.transposed   # the oracle's block order
+            op.set_eps(p.eps)
+            c = lib.fdfd_twist_block(ctypes.byref(op.grid))
+            assert c == ((m + 1) // 2 if twist else m - 1), c
+            op.factor(precision, stepwise=True)
+            blocks[twist] = (c, op.factors[precision][0].view(dt)[: m * 2 * npad * npad].clone())
+            x, rep = op.solve(torch.tensor(b, device=DEV), precision=precision, tol=1e-12)
+            print(f"twist {twist}: c = {c}, {rep.sweeps} sweeps, residual {rep.max_rel_residual:.2e}")
+            assert rep.converged and not rep.fell_back
+            xs[twist] = x.cpu().numpy()
+        finally:
+            lib.fdfd_set_twist(prev)
+    c = blocks[1][0]
+    top = c * 2 * npad * npad
+    assert torch.equal(blocks[1][1][:top], blocks[0][1][:top])
+    # the bottom chain's first block: G_{m-1} = T_{m-1}^-1
+    A = sim.A.tocsr()
+    Tl = A[(m - 1) * n:, (m - 1) * n:].toarray()
+    g = blocks[1][1][(m - 1) * 2 * npad * npad:].double().cpu().numpy().reshape(2, npad, npad)
+    Gl = g[0, :n, :n] + 1j * g[1, :n, :n]
+    d = np.linalg.norm(Gl - np.linalg.inv(Tl)) / np.linalg.norm(Gl)
+    assert d < (1e-4 if precision == "c64" else 1e-10), d
+    for q in range(2):
+        xref = solver_direct(A, b[q].reshape(-1)).reshape(p.shape)
+        for twist in (1, 0):
+            assert relerr(xs[twist][q], xref) < 1e-10
 
 
 @pytest.mark.parametrize("precision,n,pol,npml", [("c64", 200, "Ez", None), ("fp64", 96, "Hz", None),
@@ -315,23 +367,28 @@ def test_symmetric_steps_match_full_steps(precision, n, pol, npml):
     assert relerr(x.cpu().numpy(), xref) < 1e-10
 
 
-def test_bounded_waits_fall_back_when_forced():
+@pytest.mark.parametrize("N,twist", [((64, 96), 0), ((128, 64), 1)])
+def test_bounded_waits_fall_back_when_forced(N, twist):
     """With a one-poll spin limit every cross-workgroup wait gives up at once:
-    the persistent factor and sweeps must fall back and still be right."""
+    the persistent factor and sweeps must fall back and still be right (one
+    chain; twisted: the per-block sweeps of both chains and the twist block)."""
     lib = L.lib()
     rng = np.random.default_rng(12)
-    N = (64, 96)
     eps = 1 + 11.25 * (rng.random(N) < 0.5)
     sim = OracleSimulation(OMEGA, eps, 0.02, [10, 10], 'Ez')
     b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
     xref = solver_direct(sim.A, b.reshape(-1)).reshape(N)
     lib.fdfd_set_spin_cap(1)
+    prev = lib.fdfd_set_twist(twist)
     try:
         op = DeviceOperator(N, OMEGA, 0.02, [10, 10], 'Ez', 1e-6, DEV)
+        c = lib.fdfd_twist_block(ctypes.byref(op.grid))
+        assert (c < op.grid.nx - 1) == bool(twist), c
         op.set_eps(eps)
         x, rep = op.solve(torch.tensor(b, device=DEV), precision="c64", tol=1e-12)
     finally:
         lib.fdfd_set_spin_cap(0)
+        lib.fdfd_set_twist(prev)
     assert rep.converged
     assert relerr(x.cpu().numpy(), xref) < 1e-10
 
