@@ -2748,31 +2748,46 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
   T *Sig = (T *)fw.W, *Q = (T *)fw.W + 2 * np2;
   T *Z[2] = {(T *)fw.xcol, (T *)fw.xcol + 2 * np2};
   const T *Glast = G + (size_t)(m - 1) * 2 * np2;
-  // exact chains over all blocks, or the truncated Sigma chain without Z
+  // exact chains over all blocks, or the truncated Sigma chain without Z;
+  // for a twisted factorization also the mirrored truncated chain over the
+  // bottom blocks, Sigma'_{m-1-L} = G_{m-1-L},
+  // Sigma'_k = G_k + G_k XM_k Sigma'_{k-1} XP_{k-1} G_k, in the same launches
+  // (its temporaries in the second exchange buffer, free after the
+  // factorization)
   const bool exact = wrap_exact(g);
+  const bool twisted = twist_block(g) < m - 1;
   const int L = exact ? m - 1 : g->npml_x + WRAP_EXTRA;
   const T *sig_prev = G + (size_t)L * 2 * np2, *z_prev = exact ? Glast : nullptr;
+  T *SigB = (T *)fw.xcol, *QB = (T *)fw.xcol + 2 * np2;
+  const T *sigb_prev = G + (size_t)(m - 1 - L) * 2 * np2;
   int zi = 0;
-  for (int k = L - 1; k >= 0; --k) {
-    const T *Gk = G + (size_t)k * 2 * np2;
+  for (int i = 0; i < L; ++i) {
+    const int k = L - 1 - i, kb = m - L + i;   // top and bottom chain blocks
+    const T *Gk = G + (size_t)k * 2 * np2, *Gb = G + (size_t)kb * 2 * np2;
     GemmBatch<T> b1;
     memset(&b1, 0, sizeof b1);
-    b1.njobs = exact ? 2 : 1;
+    b1.njobs = 1;
     // Q = G_k (XP_k Sigma_{k+1} XM_{k+1})
     b1.job[0] = GemmJob<T>{Gk, sig_prev, nullptr, Q, op + 2 * (FDFD_OP_XP * M + (size_t)k * n),
                            op + 2 * (FDFD_OP_XM * M + (size_t)(k + 1) * n), T(1)};
     // Z_k = -G_k (XP_k Z_{k+1})
     if (exact)
-      b1.job[1] = GemmJob<T>{Gk, z_prev, nullptr, Z[zi], op + 2 * (FDFD_OP_XP * M + (size_t)k * n), nullptr,
-                             T(-1)};
+      b1.job[b1.njobs++] = GemmJob<T>{Gk, z_prev, nullptr, Z[zi], op + 2 * (FDFD_OP_XP * M + (size_t)k * n),
+                                      nullptr, T(-1)};
+    // Q' = G_kb (XM_kb Sigma'_{kb-1} XP_{kb-1})
+    if (twisted)
+      b1.job[b1.njobs++] = GemmJob<T>{Gb, sigb_prev, nullptr, QB, op + 2 * (FDFD_OP_XM * M + (size_t)kb * n),
+                                      op + 2 * (FDFD_OP_XP * M + (size_t)(kb - 1) * n), T(1)};
     hipLaunchKernelGGL(k_cgemm<T>, dim3(b1.njobs * nt * nt), dim3(GT), 0, s, b1, n, np);
-    // Sigma_k = G_k + Q G_k
+    // Sigma_k = G_k + Q G_k (and Sigma'_kb = G_kb + Q' G_kb)
     GemmBatch<T> b2;
     memset(&b2, 0, sizeof b2);
     b2.njobs = 1;
     b2.job[0] = GemmJob<T>{Q, Gk, Gk, Sig, nullptr, nullptr, T(1)};
-    hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(GT), 0, s, b2, n, np);
+    if (twisted) b2.job[b2.njobs++] = GemmJob<T>{QB, Gb, Gb, SigB, nullptr, nullptr, T(1)};
+    hipLaunchKernelGGL(k_cgemm<T>, dim3(b2.njobs * nt * nt), dim3(GT), 0, s, b2, n, np);
     sig_prev = Sig;
+    sigb_prev = SigB;
     if (exact) {
       z_prev = Z[zi];
       zi ^= 1;
@@ -2785,33 +2800,11 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
     hipLaunchKernelGGL(k_wrap_capacitance<T>, dim3((N + NT - 1) / NT, N), dim3(NT), 0, s, *g, op, Glast, sig_prev,
                        z_prev, np, K);
   } else {
-    // copies of Sigma_0 and of T^-1_{m-1,m-1}: G_{m-1}, or for a twisted
-    // factorization the mirrored truncated chain over the bottom blocks,
-    // Sigma'_{m-1-L} = G_{m-1-L}, Sigma'_k = G_k + G_k XM_k Sigma'_{k-1} XP_{k-1} G_k
+    // copies of Sigma_0 and of T^-1_{m-1,m-1} (G_{m-1}, or Sigma'_{m-1} of a
+    // twisted factorization)
     T *sig0 = (T *)wrap_sigma(g, prec, factor), *sigl = (T *)wrap_sigma_last(g, prec, factor);
     HIP_TRY(hipMemcpyAsync(sig0, sig_prev, 2 * np2 * sizeof(T), hipMemcpyDeviceToDevice, s));
-    if (twist_block(g) < m - 1) {
-      const T *sp = G + (size_t)(m - 1 - L) * 2 * np2;
-      for (int k = m - L; k <= m - 1; ++k) {
-        const T *Gk = G + (size_t)k * 2 * np2;
-        GemmBatch<T> b1;
-        memset(&b1, 0, sizeof b1);
-        b1.njobs = 1;
-        b1.job[0] = GemmJob<T>{Gk, sp, nullptr, Q, op + 2 * (FDFD_OP_XM * M + (size_t)k * n),
-                               op + 2 * (FDFD_OP_XP * M + (size_t)(k - 1) * n), T(1)};
-        hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(GT), 0, s, b1, n, np);
-        GemmBatch<T> b2;
-        memset(&b2, 0, sizeof b2);
-        b2.njobs = 1;
-        b2.job[0] = GemmJob<T>{Q, Gk, Gk, Sig, nullptr, nullptr, T(1)};
-        hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(GT), 0, s, b2, n, np);
-        sp = Sig;
-      }
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipMemcpyAsync(sigl, Sig, 2 * np2 * sizeof(T), hipMemcpyDeviceToDevice, s));
-    } else {
-      HIP_TRY(hipMemcpyAsync(sigl, Glast, 2 * np2 * sizeof(T), hipMemcpyDeviceToDevice, s));
-    }
+    HIP_TRY(hipMemcpyAsync(sigl, twisted ? sigb_prev : Glast, 2 * np2 * sizeof(T), hipMemcpyDeviceToDevice, s));
     // S = I - Sigma_0 XM_0 T^-1_{m-1,m-1} XP_{m-1} (order n)
     GemmBatch<T> b3;
     memset(&b3, 0, sizeof b3);
