@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define FDFD_ABI_VERSION 3
+#define FDFD_ABI_VERSION 4
 
 enum {
   FDFD_OK = 0,
