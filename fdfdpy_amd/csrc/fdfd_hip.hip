@@ -380,6 +380,18 @@ inline int batch_layout(FBatch<T> &b) {
   return XCDS * (b.crow + (w + XCDS - 1) / XCDS);
 }
 
+// Four consecutive elements of a 16-byte aligned row from global memory.
+__device__ inline void ld_row4(const float *p, float (&v)[4]) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 x = *reinterpret_cast<const f4 *>(p);
+  v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+}
+__device__ inline void ld_row4(const double *p, double (&v)[4]) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 x = reinterpret_cast<const d2 *>(p)[0], y = reinterpret_cast<const d2 *>(p)[1];
+  v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
+}
+
 // Four consecutive elements from LDS to global memory as 16-byte
 // non-temporal stores.
 __device__ inline void store4_nt(float *p, const float *v) {
@@ -904,21 +916,24 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   // symmetric jobs read X[p,J] (J < p) as tile (J,p), X[I,p] (I > p) as (p,I)
   const bool mirA = p > 0 && J.dsym && tj < p, mirB = p > 0 && J.dsym && ti > p;
   if (p > 0) {
-    // plain loads of the previous launch's output: all of them before any use
+    // plain loads of the previous launch's output, all of them before any
+    // use: thread t reads four consecutive elements of row t/8 of each tile
+    // as one 16-byte access per plane
     T pr[PER], pim[PER], ar[PER], ai[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int e = tid + u * NT, a = e / TB, b = e % TB;
-      pr[u] = Pslot[e];
-      pim[u] = Pslot[TB * TB + e];
-      const size_t ia = mirA ? (size_t)(tj * TB + a) * J.np + p * TB + b
-                             : (size_t)(p * TB + a) * J.np + tj * TB + b;
-      const size_t ib = mirB ? (size_t)(p * TB + a) * J.np + ti * TB + b
-                             : (size_t)(ti * TB + a) * J.np + p * TB + b;
-      ar[u] = has_a ? src[ia] : T(0);
-      ai[u] = has_a ? src[np2 + ia] : T(0);
-      br[u] = has_b ? src[ib] : T(0);
-      bi[u] = has_b ? src[np2 + ib] : T(0);
+    const int a = tid >> 3, b0 = (tid & 7) * 4;
+    ld_row4(Pslot + a * TB + b0, pr);
+    ld_row4(Pslot + TB * TB + a * TB + b0, pim);
+    const size_t ia = mirA ? (size_t)(tj * TB + a) * J.np + p * TB + b0
+                           : (size_t)(p * TB + a) * J.np + tj * TB + b0;
+    const size_t ib = mirB ? (size_t)(p * TB + a) * J.np + ti * TB + b0
+                           : (size_t)(ti * TB + a) * J.np + p * TB + b0;
+    if (has_a) {
+      ld_row4(src + ia, ar);
+      ld_row4(src + np2 + ia, ai);
+    }
+    if (has_b) {
+      ld_row4(src + ib, br);
+      ld_row4(src + np2 + ib, bi);
     }
     if (has_a && has_b) {
 #pragma unroll
@@ -931,7 +946,7 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
     }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int e = tid + u * NT, a = e / TB, b = e % TB;
+      const int b = b0 + u;
       Pbr[a][b] = pr[u];
       Pbi[a][b] = pim[u];
       if (mirA) {  // X[p,J][b][a] = -(D[32 J + a] / D[32 p + b]) X[J,p][a][b]
@@ -972,9 +987,10 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   // X[I, p] into the B buffer: from the registers (p > 0) or formed on load
   auto put_b = [&]() {
     if (p > 0) {
+      const int a = tid >> 3, b0 = (tid & 7) * 4;   // the load mapping above
 #pragma unroll
       for (int u = 0; u < PER; ++u) {
-        const int e = tid + u * NT, a = e / TB, b = e % TB;
+        const int b = b0 + u;
         if (mirB) {  // already scaled: transposed into place
           L.Br[b][a] = br[u];
           L.Bi[b][a] = bi[u];
@@ -1947,33 +1963,43 @@ template <typename T> struct GemmBatch {
 };
 
 // k tile kt of the operands of output tile (ti, tj): A[ti, kt] and the
-// scaled B[kt, tj], four elements of each plane per thread (zeros past nt)
+// scaled B[kt, tj]; thread t holds four consecutive elements of row t/8 of
+// each (one 16-byte load per plane; zeros past nt)
 template <typename T>
 __device__ __forceinline__ void gemm_fetch(const GemmJob<T> &J, int n, int np, int nt, int ti, int tj, int kt,
                                            int tid, T (&ar)[TB * TB / NT], T (&ai)[TB * TB / NT],
                                            T (&br)[TB * TB / NT], T (&bi)[TB * TB / NT]) {
+  static_assert(TB * TB / NT == 4, "four elements per thread");
   const size_t np2 = (size_t)np * np;
-  const bool live = kt < nt;
+  if (kt >= nt) {
 #pragma unroll
-  for (int u = 0; u < TB * TB / NT; ++u) {
-    const int e = tid + u * NT, a = e / TB, b = e % TB;
-    const int brow = kt * TB + a, bcol = tj * TB + b;
-    const size_t ia = (size_t)(ti * TB + a) * np + kt * TB + b, ib = (size_t)brow * np + bcol;
-    ar[u] = live ? J.A[ia] : T(0);
-    ai[u] = live ? J.A[np2 + ia] : T(0);
-    double xr = live ? (double)J.B[ib] : 0.0, xi = live ? (double)J.B[np2 + ib] : 0.0;
-    if (J.rs && live) {
-      const cd sc = brow < n ? ld_c(J.rs, brow) : cd{0, 0};
-      const double yr = sc.r * xr - sc.i * xi, yi = sc.r * xi + sc.i * xr;
-      xr = yr; xi = yi;
+    for (int u = 0; u < 4; ++u) { ar[u] = ai[u] = br[u] = bi[u] = T(0); }
+    return;
+  }
+  const int a = tid >> 3, b0 = (tid & 7) * 4;
+  const int brow = kt * TB + a;
+  const size_t ia = (size_t)(ti * TB + a) * np + kt * TB + b0, ib = (size_t)brow * np + tj * TB + b0;
+  ld_row4(J.A + ia, ar);
+  ld_row4(J.A + np2 + ia, ai);
+  T xr[4], xi[4];
+  ld_row4(J.B + ib, xr);
+  ld_row4(J.B + np2 + ib, xi);
+  const cd rsc = J.rs ? (brow < n ? ld_c(J.rs, brow) : cd{0, 0}) : cd{1, 0};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    double yr = xr[u], yi = xi[u];
+    if (J.rs) {
+      const double zr = rsc.r * yr - rsc.i * yi, zi = rsc.r * yi + rsc.i * yr;
+      yr = zr; yi = zi;
     }
-    if (J.cs && live) {
+    if (J.cs) {
+      const int bcol = tj * TB + b0 + u;
       const cd sc = bcol < n ? ld_c(J.cs, bcol) : cd{0, 0};
-      const double yr = sc.r * xr - sc.i * xi, yi = sc.r * xi + sc.i * xr;
-      xr = yr; xi = yi;
+      const double zr = sc.r * yr - sc.i * yi, zi = sc.r * yi + sc.i * yr;
+      yr = zr; yi = zi;
     }
-    br[u] = (T)xr;
-    bi[u] = (T)xi;
+    br[u] = (T)yr;
+    bi[u] = (T)yi;
   }
 }
 
@@ -1999,8 +2025,8 @@ __global__ void __launch_bounds__(GT) k_cgemm(GemmBatch<T> batch, int n, int np)
   gemm_fetch<T>(J, n, np, nt, ti, tj, grp, tid, ar, ai, br, bi);
   for (int it = 0; it < iters; ++it) {
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int e = tid + u * NT, a = e / TB, b = e % TB;
+    for (int u = 0; u < PER; ++u) {   // gemm_fetch's mapping: row tid/8, columns 4 (tid % 8) + u
+      const int a = tid >> 3, b = (tid & 7) * 4 + u;
       Ar[grp][a][b] = ar[u];
       Ai[grp][a][b] = ai[u];
       Br[grp][a][b] = br[u];
@@ -2033,7 +2059,7 @@ __device__ inline cd stretch_d(const fdfd_grid &g, int i, int j) {
 }
 
 // K = I + V^T T^-1 U C as a row-major complex FP64 matrix of order 2n
-constexpr int WRAP_MIN_PML = 4, WRAP_EXTRA = 8;
+constexpr int WRAP_MIN_PML = 4, WRAP_EXTRA = 4;
 
 // Y = NULL: the far corners are taken as zero
 template <typename T>
