@@ -44,11 +44,12 @@ def _device_solve(omega, eps, src, npml, **kw):
 
 
 def test_c64_stall_falls_back_to_fp64():
-    """tol 1e-12 in 3 refinement steps: the complex64 factor needs about 4
-    steps for that, the FP64 factor 3 (lagged truncated wrap correction: the
-    first step leaves the wrap coupling, the second corrects it)."""
+    """tol 1e-13 in 3 refinement steps: the complex64 factor reaches 1e-11 to
+    1e-12 in 3 steps and needs a fourth, the FP64 factor gets to about 4e-15
+    in 3 (lagged truncated wrap correction: the first step leaves the wrap
+    coupling, the second corrects it)."""
     omega, eps, src, npml = _problem()
-    x, rep = _device_solve(omega, eps, src, npml, precision='c64', tol=1e-12, max_iter=3)
+    x, rep = _device_solve(omega, eps, src, npml, precision='c64', tol=1e-13, max_iter=3)
     print(f"c64 -> {rep.precision}: {rep.iterations} steps, residual {rep.max_rel_residual:.2e}")
     assert rep.converged and rep.fell_back and rep.precision == 'fp64' and not rep.pivoted
     _, xr = _oracle_x(omega, eps, src, npml)
