@@ -335,6 +335,7 @@ constexpr int XCDS = 8;
 template <typename T> struct FBatch {
   int njobs;
   int crow;                // ceil(njobs / XCDS)
+  int total;               // first[njobs]: the non-critical tiles of all jobs
   int first[MAXJ + 1];
   FJob<T> job[MAXJ];
 };
@@ -362,11 +363,16 @@ template <typename T>
 __device__ inline int step_tile_of(const FJob<T> &J, int r) {
   const int c = critical_tile(J), nt = J.nt;
   if (!J.dsym) return r < c ? r : r + 1;
-  // tiles I <= J in row order; the critical tile (on the diagonal) skipped
+  // tiles I <= J in row order; the critical tile (on the diagonal) skipped.
+  // Row I starts at S(I) = I nt - I (I - 1) / 2: I is the larger root of
+  // S(I) = u, rounded down, then corrected by one either way for rounding.
   const int ci = c / nt, cu = ci * nt - ci * (ci - 1) / 2;
-  int u = r < cu ? r : r + 1, I = 0;
-  while (u >= nt - I) { u -= nt - I; ++I; }
-  return I * nt + I + u;
+  const int u = r < cu ? r : r + 1;
+  const float h = nt + 0.5f;
+  int I = (int)(h - sqrtf(h * h - 2.0f * u));
+  if (I * nt - I * (I - 1) / 2 > u) --I;
+  else if (I + 1 < nt && (I + 1) * nt - (I + 1) * I / 2 <= u) ++I;
+  return I * nt + u - (I * nt - I * (I - 1) / 2) + I;
 }
 template <typename T>
 inline int batch_layout(FBatch<T> &b) {
@@ -376,8 +382,19 @@ inline int batch_layout(FBatch<T> &b) {
     w += step_tiles(b.job[j]) - 1;
   }
   b.first[b.njobs] = w;
+  b.total = w;
   b.crow = (b.njobs + XCDS - 1) / XCDS;
   return XCDS * (b.crow + (w + XCDS - 1) / XCDS);
+}
+
+// An empty asm statement that takes a job's fields in scalar registers: the
+// reads of all of them issue before it and complete in one wait, instead of
+// one round trip each where the compiler would wait on them behind the
+// branches that depend on earlier ones.
+template <typename T>
+__device__ inline void in_sgprs(const FJob<T> &J) {
+  asm volatile("" ::"s"(J.op), "s"(J.G), "s"(J.W), "s"(J.Pinv), "s"(J.status), "s"(J.dense), "s"(J.dsym),
+               "s"(J.m), "s"(J.n), "s"(J.np), "s"(J.nt), "s"(J.k), "s"(J.p), "s"(J.dir), "s"(J.c));
 }
 
 // Four consecutive elements of a 16-byte aligned row from global memory.
@@ -494,6 +511,7 @@ template <typename T> struct LdsStep {
   T Ar[TB][TBP], Ai[TB][TBP];  // X[p,J], then R; or P (column p); tile to invert / inverse
   T Br[TB][TBP], Bi[TB][TBP];  // P, then X[I,p]
   unsigned char pad[step_lds_pad<T>()];
+  alignas(16) T dmir[4][TB];   // D of the tile's rows (re, im), 1/D of its columns (re, im)
   T mag[2][TB];
   int perm[TB];
   int colmap[TB];
@@ -840,14 +858,6 @@ __device__ void block_symmetrizer(const FJob<T> &J) {
   }
 }
 
-// sign * D[i] / D[j] of a symmetric job
-template <typename T>
-__device__ inline void sym_ratio(const T *dsym, int np, int i, int j, T sign, T &rr, T &ri) {
-  const T ar = dsym[i], ai = dsym[np + i], br = dsym[2 * np + j], bi = dsym[3 * np + j];
-  rr = sign * (ar * br - ai * bi);
-  ri = sign * (ar * bi + ai * br);
-}
-
 // One Gauss-Jordan step (k, p) for every job of the batch; one output tile
 // (I, J) per workgroup:
 //   I == p, J == p : P                     (P = inverse of X[p,p])
@@ -863,19 +873,27 @@ __device__ inline void sym_ratio(const T *dsym, int np, int i, int j, T sign, T 
 template <typename T, bool PIV>
 __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   __shared__ LdsStep<T> L;
-  int jb = 0, local = 0;
-  if ((int)blockIdx.x < batch.njobs) {
-    jb = blockIdx.x;
-    local = critical_tile(batch.job[jb]);
-  } else {
-    const int b = blockIdx.x, x = b % XCDS, i = b / XCDS - batch.crow, w = batch.first[batch.njobs];
-    if (i < 0) return;  // idle slot beside the critical workgroups
-    const int g = (x * w) / XCDS + i;
-    if (g >= ((x + 1) * w) / XCDS) return;
-    while (jb + 1 < batch.njobs && g >= batch.first[jb + 1]) ++jb;
-    local = step_tile_of(batch.job[jb], g - batch.first[jb]);
-  }
+  // The job of this workgroup. The batch lives in the kernel arguments. Lane
+  // j of each wave reads first[j] (one vector read beside the header's
+  // scalar reads); a ballot then gives the last job whose tiles start at or
+  // before this workgroup's, and the job's fields follow in one more round
+  // trip. (A scan with the job index in a scalar register waited for one
+  // read per job; an unrolled scalar search kept the CU's scalar unit busy.)
+  const int lane0 = threadIdx.x & 63;
+  const int njobs = batch.njobs, crow = batch.crow, wtot = batch.total;
+  const int fl = batch.first[lane0 < MAXJ ? lane0 : 0];
+  const int bid = blockIdx.x, xcd = bid % XCDS, row = bid / XCDS - crow;
+  const bool crit = bid < njobs;
+  const int g = (xcd * wtot) / XCDS + row;
+  const unsigned long long hit = __ballot(lane0 < njobs && fl <= g);
+  const int jn = hit ? 63 - __clzll(hit) : 0;
+  const int base = __builtin_amdgcn_readlane(fl, jn);
+  // idle slots beside the critical workgroups, and the end of each range
+  if (!crit && (row < 0 || g >= ((xcd + 1) * wtot) / XCDS)) return;
+  const int jb = crit ? bid : jn;
   const FJob<T> &J = batch.job[jb];
+  in_sgprs(J);
+  const int local = crit ? critical_tile(J) : step_tile_of(J, g - base);
   // the critical workgroups bound the launch: their waves issue first on
   // the CUs they share with the others
   if ((int)blockIdx.x < batch.njobs) __builtin_amdgcn_s_setprio(3);
@@ -944,14 +962,26 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
         ci[t] = src[np2 + idx];
       }
     }
+    // the symmetrizer entries of a mirrored tile, read with the tiles: D of
+    // row 32 R + a and 1/D of columns 32 C + b0.. (R, C = J, p for X[p,J];
+    // p, I for X[I,p]; at most one of the two is mirrored when I <= J)
+    T dr = 0, di = 0, er[PER], ei[PER];
+    if (mirA || mirB) {
+      const int R = mirA ? tj : p, C = mirA ? p : ti;
+      dr = J.dsym[R * TB + a];
+      di = J.dsym[J.np + R * TB + a];
+      ld_row4(J.dsym + 2 * J.np + C * TB + b0, er);
+      ld_row4(J.dsym + 3 * J.np + C * TB + b0, ei);
+    }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int b = b0 + u;
       Pbr[a][b] = pr[u];
       Pbi[a][b] = pim[u];
+      // sign * D[32 R + a] / D[32 C + b]
+      const T sg = mirA ? T(-1) : T(1);
+      const T rr = sg * (dr * er[u] - di * ei[u]), ri = sg * (dr * ei[u] + di * er[u]);
       if (mirA) {  // X[p,J][b][a] = -(D[32 J + a] / D[32 p + b]) X[J,p][a][b]
-        T rr, ri;
-        sym_ratio(J.dsym, J.np, tj * TB + a, p * TB + b, T(-1), rr, ri);
         L.Ar[b][a] = rr * ar[u] - ri * ai[u];
         L.Ai[b][a] = rr * ai[u] + ri * ar[u];
       } else if (has_a) {
@@ -959,8 +989,6 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
         L.Ai[a][b] = ai[u];
       }
       if (mirB) {  // X[I,p][b][a] = (D[32 p + a] / D[32 I + b]) X[p,I][a][b], scaled now
-        T rr, ri;
-        sym_ratio(J.dsym, J.np, p * TB + a, ti * TB + b, T(1), rr, ri);
         const T xr = rr * br[u] - ri * bi[u], xi = rr * bi[u] + ri * br[u];
         br[u] = xr;
         bi[u] = xi;
@@ -982,6 +1010,23 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
         cr[t] = xr;
         ci[t] = xi;
       }
+    }
+  }
+  // the symmetrizer entries of the tile's mirror image, stored at a block's
+  // last step, read with the tiles and kept in LDS until the store: D of the
+  // tile's rows 32 I.. (threads 0-7) and 1/D of its columns 32 J.. (threads
+  // 0, 8, .., 248)
+  if (J.dsym && p == nt - 1 && ti != tj) {
+    if (tid < 8) {
+      T fr[4], fi[4];
+      ld_row4(J.dsym + ti * TB + 4 * tid, fr);
+      ld_row4(J.dsym + J.np + ti * TB + 4 * tid, fi);
+      lds_st4(&L.dmir[0][4 * tid], fr);
+      lds_st4(&L.dmir[1][4 * tid], fi);
+    }
+    if ((tid & 7) == 0) {
+      L.dmir[2][tid >> 3] = J.dsym[2 * J.np + tj * TB + (tid >> 3)];
+      L.dmir[3][tid >> 3] = J.dsym[3 * J.np + tj * TB + (tid >> 3)];
     }
   }
   // X[I, p] into the B buffer: from the registers (p > 0) or formed on load
@@ -1054,8 +1099,9 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
       T mr[4], mi[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        T rr, ri;
-        sym_ratio(J.dsym, J.np, ti * TB + c0 + q, tj * TB + i, T(1), rr, ri);
+        // D[32 I + c0 + q] / D[32 J + i]
+        const T fr = L.dmir[0][c0 + q], fi = L.dmir[1][c0 + q], gr = L.dmir[2][i], gi = L.dmir[3][i];
+        const T rr = fr * gr - fi * gi, ri = fr * gi + fi * gr;
         const T vr = L.Ar[c0 + q][i], vi = L.Ai[c0 + q][i];
         mr[q] = rr * vr - ri * vi;
         mi[q] = rr * vi + ri * vr;
@@ -1943,7 +1989,7 @@ __global__ void __launch_bounds__(NT) k_psweep_ll(const T *__restrict__ G, const
 // L = NPML_x + WRAP_EXTRA, past the x-PML, with Sigma_L = G_L. What the
 // truncation leaves out is the part of T^-1_{0,0} reflected from beyond L,
 // which has crossed the x-PML twice and does not shrink with a longer chain;
-// the refinement takes as many steps with L = NPML_x + 8 as with
+// the refinement takes as many steps with L = NPML_x + 4 as with
 // 2 NPML_x + 16. The chains shrink from 3 (m - 1) products to 2 L. Without
 // such PMLs (periodic axes, where waves along the blocks are never absorbed)
 // the chains run over all blocks and K is exact.
