@@ -348,8 +348,9 @@ def _cpu_solve(task):
         p = C.c3_request(seed)
         t = time.perf_counter()
         A, _ = construct_A(p.omega, [0, p.shape[0] * p.dl], [0, p.shape[1] * p.dl], p.eps, p.npml, p.pol, p.L0)
-        solver_direct(A, p.srcs[q] * 1j * p.omega)
-        return seed, q, time.perf_counter() - t, time.perf_counter()
+        x = solver_direct(A, p.srcs[q] * 1j * p.omega)
+        t1 = time.perf_counter()
+        return seed, q, t1 - t, t1, x
 
 
 def cpu_baseline(reqs, seconds, workers=0):
@@ -387,11 +388,13 @@ def cpu_baseline(reqs, seconds, workers=0):
                 time.sleep(wait)
             for q in range(r.problem.srcs.shape[0]):
                 asyncs.append((r, pool.apply_async(_cpu_solve, ((r.rid, q),))))
-        done = {}
+        done, sols = {}, {}
         for r, a in asyncs:
-            seed, q, dt, tend = a.get()
+            seed, q, dt, tend, x = a.get()
             done.setdefault(r.rid, []).append((dt, tend))
-        wall = time.perf_counter() - t0
+            sols[(r.rid, q)] = x
+        # to the last solve's end (the solutions' transfer back is not timed)
+        wall = max(te for v in done.values() for _, te in v) - t0
     lat, svc = [], []
     for r in sample:
         ends = [te for _, te in done[r.rid]]
@@ -406,7 +409,35 @@ def cpu_baseline(reqs, seconds, workers=0):
             "p50_latency_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
             "p99_latency_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
             "p50_service_ms": round(float(np.percentile(svc, 50)) * 1e3, 1),
-            "p99_service_ms": round(float(np.percentile(svc, 99)) * 1e3, 1)}
+            "p99_service_ms": round(float(np.percentile(svc, 99)) * 1e3, 1)}, sols
+
+
+def result_errors(res, ref):
+    """Relative L2 error of each result (request, RHS) of `res` against
+    `ref`: {(rid, q): solution} (device tensors or flattened numpy arrays of
+    the same grid order). Returns {(rid, q): error}."""
+    import torch
+    out = {}
+    for r in res:
+        if r.fields is None:
+            continue
+        for q in range(r.nrhs):
+            y = ref.get((r.rid, q))
+            if y is None:
+                continue
+            x = r.fields[q].reshape(-1)
+            y = torch.as_tensor(y, device=x.device).reshape(-1)
+            out[(r.rid, q)] = float(torch.linalg.norm(x - y) / torch.linalg.norm(y))
+    return out
+
+
+def _error_summary(errs, reference):
+    import numpy as np
+    v = np.array(list(errs.values()))
+    if not len(v):
+        return None
+    return {"results": int(len(v)), "max": float(v.max()), "p99": float(np.percentile(v, 99)),
+            "p50": float(np.percentile(v, 50)), "reference": reference}
 
 
 def _sync_time(fn):
@@ -491,10 +522,10 @@ def config_timings():
     return out
 
 
-def run_stream(reqs, precision, slots, tol, engines=2):
+def run_stream(reqs, precision, slots, tol, engines=2, keep=False):
     import torch
     from fdfdpy_amd.serving import ServingEngine, summarize
-    eng = ServingEngine("cuda:0", precision=precision, slots=slots, tol=tol, engines=engines)
+    eng = ServingEngine("cuda:0", precision=precision, slots=slots, tol=tol, engines=engines, keep_fields=keep)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
 
@@ -554,10 +585,10 @@ def main():
         log("traffic: two rocprofv3 --pmc passes over the roofline leg's launches")
         traffic = traffic_leg()
         log(f"traffic: {json.dumps(traffic)}")
-    cpu = None
+    cpu, cpu_sols = None, {}
     if not (args.no_cpu_baseline or only):
         log("cpu: pool replay of a prefix of the stream")
-        cpu = cpu_baseline(reqs, args.cpu_seconds, args.cpu_workers)
+        cpu, cpu_sols = cpu_baseline(reqs, args.cpu_seconds, args.cpu_workers)
         log(f"cpu: {json.dumps(cpu)}")
 
     import torch
@@ -582,13 +613,28 @@ def main():
             s, _ = run_stream(wreqs, prec, args.slots, args.tol, args.engines)
             log(f"warmup {w} {prec}: {s['solves']} solves in {s['wall_s']:.2f}s")
 
-    # throughput leg: the whole stream, offered faster than it is served
-    mixed, _ = run_stream(reqs, "c64", args.slots, args.tol, args.engines)
+    # throughput leg: the whole stream, offered faster than it is served; the
+    # results are kept (outside the timing) for the per-result errors
+    mixed, res_m = run_stream(reqs, "c64", args.slots, args.tol, args.engines, keep=True)
     log(f"mixed: {json.dumps(mixed)}")
     fp64 = None
+    errors = {}
+    if cpu_sols:
+        e = result_errors(res_m, cpu_sols)
+        errors["vs_oracle"] = _error_summary(e, "the oracle's FP64 SuperLU solve of the same request and RHS "
+                                                "(the CPU baseline's sample)")
+        if errors["vs_oracle"]:
+            errors["vs_oracle"]["per_result"] = {f"{rid}.{q}": float(f"{v:.3g}") for (rid, q), v in sorted(e.items())}
     if not args.no_fp64:
-        fp64, _ = run_stream(reqs, "fp64", args.slots, args.tol, args.engines)
+        fp64, res_f = run_stream(reqs, "fp64", args.slots, args.tol, args.engines, keep=True)
         log(f"fp64: {json.dumps(fp64)}")
+        ref = {(r.rid, q): r.fields[q] for r in res_f if r.fields is not None for q in range(r.nrhs)}
+        errors["vs_fp64_path"] = _error_summary(result_errors(res_m, ref),
+                                                "the FP64-factor path's result of the same request and RHS")
+        del res_f, ref
+    del res_m
+    if errors:
+        log(f"errors: {json.dumps({k_: {a: b for a, b in v.items() if a != 'per_result'} for k_, v in errors.items() if v})}")
     if args.stream_only:
         print(json.dumps({"stream_c64": mixed, "stream_fp64": fp64}), flush=True)
         return
@@ -633,6 +679,7 @@ def main():
         "max_rel_residual": max(mixed["max_rel_residual"], lat_m["max_rel_residual"]),
         "fallbacks": mixed["fallbacks"], "failed": mixed["failed"] + lat_m["failed"],
         "mean_steps": mixed["mean_steps"], "mean_sweeps": mixed["mean_sweeps"],
+        "error_per_result": errors or None,
         "saturated": _path_summary(mixed) | {"engine_launches": mixed.get("engine_launches"),
                                              "jobs_per_launch": mixed.get("jobs_per_launch")},
         "roofline": {k_: roof[k_] for k_ in ("bound", "achieved", "peak", "unit", "frac", "traffic")},

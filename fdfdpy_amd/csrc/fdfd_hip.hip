@@ -314,8 +314,10 @@ template <typename T> struct FJob {
   T *G;              // [m][2][np][np] planar (re plane, im plane) inverses
   T *W;              // [2][np][np] ping-pong partner of G_k
   T *Pinv;           // [2 slots][2][TB][TB]
-  int *status;       // set to 1 on a non-finite / zero pivot
+  int *status;       // set to 1 on a non-finite / zero pivot, 2 on a timed-out wait
   const T *dense;    // one dense matrix ([2][np][np]) to invert (m = 1); null: the operator's S_k
+  T *rbuf;           // [nt][2][TB][TB] R = P X[p,J] of the step, by column J (symmetric jobs)
+  unsigned *rflag;   // [nt] the chain step + 1 whose R[J] is in rbuf; null: no column workgroups
   T *dsym;           // [4][np] D, 1/D (re, im planes) of the block symmetrizer, written
                      // by the initial-pivot launch; null: every tile of every step
   int m, n, np, nt;  // blocks, block size, padded block size, tiles per edge
@@ -326,17 +328,28 @@ template <typename T> struct FJob {
 };
 // Workgroup b < njobs takes the critical tile of job b (the tile inverted for
 // the next step), so those start first; the rest of the first crow * XCDS
-// workgroups idle. The other tiles of all jobs, concatenated (job j's are
-// [first[j], first[j + 1]), in row order), are cut into XCDS equal ranges and
-// range x goes to the workgroups b = XCDS * (crow + i) + x, which the
-// dispatcher places on XCD x: each XCD reads a job's pivot-row and pivot-
-// column tiles into its own L2 only for the jobs (or rows of a job) it holds.
+// workgroups idle. Then come the column workgroups of symmetric jobs (job j's
+// are [rfirst[j], rfirst[j + 1]) after XCDS * crow): one per column J != p,
+// each forms R[J] = P X[p,J] once for all the tiles of its column. The other
+// tiles of all jobs, concatenated (job j's are [first[j], first[j + 1]), in
+// row order), are cut into XCDS equal ranges and range x goes to the
+// workgroups b = nbase + XCDS i + x, which the dispatcher places on XCD x:
+// each XCD reads a job's pivot-row and pivot-column tiles into its own L2
+// only for the jobs (or rows of a job) it holds. A tile workgroup that needs
+// R[J] waits for its column workgroup's flag; the dispatcher starts the
+// workgroups of each XCD in index order, so every column workgroup has
+// started before any tile workgroup of its XCD, and column workgroups wait
+// for nothing.
 constexpr int XCDS = 8;
 template <typename T> struct FBatch {
   int njobs;
   int crow;                // ceil(njobs / XCDS)
-  int total;               // first[njobs]: the non-critical tiles of all jobs
+  int total;               // first[njobs]: the tile workgroups of all jobs
+  int rtotal;              // rfirst[njobs]: the column workgroups of all jobs
+  int nbase;               // first tile workgroup: XCDS * (crow + ceil(rtotal / XCDS))
+  unsigned cap;            // polls of a bounded wait before it gives up
   int first[MAXJ + 1];
+  int rfirst[MAXJ + 1];
   FJob<T> job[MAXJ];
 };
 // Tiles per launch of a job, and its critical tile: (p+1, p+1), or (0, 0)
@@ -358,33 +371,68 @@ template <typename T>
 __host__ __device__ inline int critical_tile(const FJob<T> &J) {
   return (J.k >= 0 && J.p + 1 < J.nt) ? (J.p + 1) * (J.nt + 1) : 0;
 }
-// Tile (I * nt + J) of the r-th non-critical workgroup of a job.
+// Column workgroups: a symmetric job (after its initial pivot, with a
+// column buffer) forms R[J] = P X[p,J] for every J != p in workgroups of
+// their own; its tiles of row p (J > p) are then those workgroups' output.
 template <typename T>
-__device__ inline int step_tile_of(const FJob<T> &J, int r) {
-  const int c = critical_tile(J), nt = J.nt;
-  if (!J.dsym) return r < c ? r : r + 1;
-  // tiles I <= J in row order; the critical tile (on the diagonal) skipped.
-  // Row I starts at S(I) = I nt - I (I - 1) / 2: I is the larger root of
-  // S(I) = u, rounded down, then corrected by one either way for rounding.
-  const int ci = c / nt, cu = ci * nt - ci * (ci - 1) / 2;
-  const int u = r < cu ? r : r + 1;
+__host__ __device__ inline bool uses_columns(const FJob<T> &J) {
+  return J.dsym && J.rflag && J.k >= 0 && J.nt > 1;
+}
+template <typename T>
+__host__ __device__ inline int column_wgs(const FJob<T> &J) {
+  return uses_columns(J) ? J.nt - 1 : 0;
+}
+// Tile workgroups of a job: its tiles but the critical one and, with column
+// workgroups, those of row p right of the diagonal.
+template <typename T>
+__host__ __device__ inline int tile_wgs(const FJob<T> &J) {
+  return step_tiles(J) - 1 - (uses_columns(J) ? J.nt - 1 - J.p : 0);
+}
+// Tile (I * nt + J) of the u-th tile I <= J of an nt x nt tile grid in row
+// order. Row I starts at S(I) = I nt - I (I - 1) / 2: I is the larger root
+// of S(I) = u, rounded down, then corrected by one either way for rounding.
+__device__ inline int upper_tile(int nt, int u) {
   const float h = nt + 0.5f;
   int I = (int)(h - sqrtf(h * h - 2.0f * u));
   if (I * nt - I * (I - 1) / 2 > u) --I;
   else if (I + 1 < nt && (I + 1) * nt - (I + 1) * I / 2 <= u) ++I;
   return I * nt + u - (I * nt - I * (I - 1) / 2) + I;
 }
+// Tile of the r-th tile workgroup of a job.
+template <typename T>
+__device__ inline int step_tile_of(const FJob<T> &J, int r) {
+  const int c = critical_tile(J), nt = J.nt, p = J.p;
+  if (!J.dsym) return r < c ? r : r + 1;
+  if (uses_columns(J)) {
+    // row order with (p, p+1) .. (p, nt-1) and the critical (p+1, p+1) taken
+    // out (one run after (p, p) at S(p)); at p = nt - 1 only (0, 0) is out
+    if (p == nt - 1) return upper_tile(nt, r + 1);
+    const int sp = p * nt - p * (p - 1) / 2;
+    return upper_tile(nt, r <= sp ? r : r + nt - p);
+  }
+  // row order with the critical tile (on the diagonal) taken out
+  const int ci = c / nt, cu = ci * nt - ci * (ci - 1) / 2;
+  return upper_tile(nt, r < cu ? r : r + 1);
+}
 template <typename T>
 inline int batch_layout(FBatch<T> &b) {
-  int w = 0;
+  int w = 0, r = 0;
   for (int j = 0; j < b.njobs; ++j) {
     b.first[j] = w;
-    w += step_tiles(b.job[j]) - 1;
+    b.rfirst[j] = r;
+    w += tile_wgs(b.job[j]);
+    r += column_wgs(b.job[j]);
   }
   b.first[b.njobs] = w;
+  b.rfirst[b.njobs] = r;
   b.total = w;
+  b.rtotal = r;
   b.crow = (b.njobs + XCDS - 1) / XCDS;
-  return XCDS * (b.crow + (w + XCDS - 1) / XCDS);
+  b.nbase = XCDS * (b.crow + (r + XCDS - 1) / XCDS);
+  // the column waits keep the default bound: the spin-cap setting shortens
+  // the persistent kernels' waits, which have a fallback; these have none
+  b.cap = SPIN_CAP;
+  return b.nbase + XCDS * ((w + XCDS - 1) / XCDS);
 }
 
 // An empty asm statement that takes a job's fields in scalar registers: the
@@ -394,7 +442,8 @@ inline int batch_layout(FBatch<T> &b) {
 template <typename T>
 __device__ inline void in_sgprs(const FJob<T> &J) {
   asm volatile("" ::"s"(J.op), "s"(J.G), "s"(J.W), "s"(J.Pinv), "s"(J.status), "s"(J.dense), "s"(J.dsym),
-               "s"(J.m), "s"(J.n), "s"(J.np), "s"(J.nt), "s"(J.k), "s"(J.p), "s"(J.dir), "s"(J.c));
+               "s"(J.rbuf), "s"(J.rflag), "s"(J.m), "s"(J.n), "s"(J.np), "s"(J.nt), "s"(J.k), "s"(J.p),
+               "s"(J.dir), "s"(J.c));
 }
 
 // Four consecutive elements of a 16-byte aligned row from global memory.
@@ -858,6 +907,118 @@ __device__ void block_symmetrizer(const FJob<T> &J) {
   }
 }
 
+// Thread 0 waits until column J's flag holds the step's tag (or the job's
+// status is set: the factorization has failed anyway); then the whole
+// group. After `cap` polls the wait gives up and sets the status to 2, so a
+// grid always drains.
+template <typename T>
+__device__ inline void wait_column(const FJob<T> &J, int col, unsigned tag, unsigned cap) {
+  if (threadIdx.x == 0) {
+    unsigned spins = 0;
+    while (ld_xcd(J.rflag + col) < tag) {
+      if ((++spins & 63) == 0 && ld_xcd(J.status)) break;
+      if (spins > cap) {
+        st_xcd(J.status, 2);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+}
+
+// The same wait by every wave on its own (all lanes read the one word), so
+// a wave goes on to read R[J] without a barrier that would also wait for
+// the group's other reads in flight.
+template <typename T>
+__device__ inline void wave_wait_column(const FJob<T> &J, int col, unsigned tag, unsigned cap) {
+  unsigned spins = 0;
+  while (ld_xcd(J.rflag + col) < tag) {
+    if ((++spins & 63) == 0 && ld_xcd(J.status)) break;
+    if (spins > cap) {
+      if ((threadIdx.x & 63) == 0) st_xcd(J.status, 2);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// The column workgroup r of a job: R[J] = P X[p,J] for column J = r (r < p)
+// or r + 1, into the job's column buffer (agent-coherent stores, then the
+// flag); right of the diagonal R[J] is also the step's result for tile
+// (p, J). X[p,J] is read like the tiles' X[p,J]: as the previous output,
+// through the symmetry for J < p, or formed from the operator at p = 0.
+template <typename T>
+__device__ void column_step(const FJob<T> &J, LdsStep<T> &L, int r, const T *src, T *dst, const T *Pslot,
+                            unsigned tag) {
+  const int p = J.p, col = r < p ? r : r + 1;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, qi = w >> 1, qj = w & 1;
+  const size_t np2 = (size_t)J.np * J.np;
+  constexpr int PER = TB * TB / NT;
+  const int a = tid >> 3, b0 = (tid & 7) * 4;
+  if (p > 0) {
+    const bool mir = col < p;   // X[p,J] = -(D[32 J + a] / D[32 p + b]) X[J,p]^T
+    T pr[PER], pim[PER], xr[PER], xi[PER], er[PER], ei[PER], dr = 0, di = 0;
+    ld_row4(Pslot + a * TB + b0, pr);
+    ld_row4(Pslot + TB * TB + a * TB + b0, pim);
+    const size_t ia = mir ? (size_t)(col * TB + a) * J.np + p * TB + b0 : (size_t)(p * TB + a) * J.np + col * TB + b0;
+    ld_row4(src + ia, xr);
+    ld_row4(src + np2 + ia, xi);
+    if (mir) {
+      dr = J.dsym[col * TB + a];
+      di = J.dsym[J.np + col * TB + a];
+      ld_row4(J.dsym + 2 * J.np + p * TB + b0, er);
+      ld_row4(J.dsym + 3 * J.np + p * TB + b0, ei);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      L.Br[a][b0 + u] = pr[u];
+      L.Bi[a][b0 + u] = pim[u];
+      if (mir) {
+        const T rr = -(dr * er[u] - di * ei[u]), ri = -(dr * ei[u] + di * er[u]);
+        L.Ar[b0 + u][a] = rr * xr[u] - ri * xi[u];
+        L.Ai[b0 + u][a] = rr * xi[u] + ri * xr[u];
+      } else {
+        L.Ar[a][b0 + u] = xr[u];
+        L.Ai[a][b0 + u] = xi[u];
+      }
+    }
+  } else {
+    for (int e = tid; e < TB * TB; e += NT) {
+      L.Br[e / TB][e % TB] = Pslot[e];
+      L.Bi[e / TB][e % TB] = Pslot[TB * TB + e];
+    }
+    load_tile(J, J.k, 0, src, 0, col, L.Ar, L.Ai);
+  }
+  __syncthreads();
+  typename Mfma<T>::v4 rr = {0, 0, 0, 0}, ri = {0, 0, 0, 0};
+  cgemm_quad<T>(rr, ri, L.Br, L.Bi, L.Ar, L.Ai, qi, qj, lane, T(1));
+  T *R = J.rbuf + (size_t)col * 2 * TB * TB;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int ra = qi * 16 + Mfma<T>::crow(lane, t), rb = qj * 16 + (lane & 15);
+    st_xcd(R + ra * TB + rb, rr[t]);
+    st_xcd(R + TB * TB + ra * TB + rb, ri[t]);
+  }
+  if (col > p) {
+    // tile (p, J) of the step's output: whole rows through LDS
+    __syncthreads();   // every wave has read X[p,J]
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int ra = qi * 16 + Mfma<T>::crow(lane, t), rb = qj * 16 + (lane & 15);
+      L.Ar[ra][rb] = rr[t];
+      L.Ai[ra][rb] = ri[t];
+    }
+    __syncthreads();
+    const size_t idx = (size_t)(p * TB + a) * J.np + col * TB + b0;
+    store4_nt(dst + idx, &L.Ar[a][b0]);
+    store4_nt(dst + np2 + idx, &L.Ai[a][b0]);
+  }
+  stores_complete();
+  __syncthreads();
+  if (tid == 0) st_xcd(J.rflag + col, tag);
+}
+
 // One Gauss-Jordan step (k, p) for every job of the batch; one output tile
 // (I, J) per workgroup:
 //   I == p, J == p : P                     (P = inverse of X[p,p])
@@ -874,36 +1035,41 @@ template <typename T, bool PIV>
 __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   __shared__ LdsStep<T> L;
   // The job of this workgroup. The batch lives in the kernel arguments. Lane
-  // j of each wave reads first[j] (one vector read beside the header's
-  // scalar reads); a ballot then gives the last job whose tiles start at or
-  // before this workgroup's, and the job's fields follow in one more round
-  // trip. (A scan with the job index in a scalar register waited for one
-  // read per job; an unrolled scalar search kept the CU's scalar unit busy.)
+  // j of each wave reads first[j] and rfirst[j] (vector reads beside the
+  // header's scalar reads); a ballot then gives the last job whose
+  // workgroups start at or before this one, and the job's fields follow in
+  // one more round trip. (A scan with the job index in a scalar register
+  // waited for one read per job; an unrolled scalar search kept the CU's
+  // scalar unit busy.)
   const int lane0 = threadIdx.x & 63;
-  const int njobs = batch.njobs, crow = batch.crow, wtot = batch.total;
-  const int fl = batch.first[lane0 < MAXJ ? lane0 : 0];
-  const int bid = blockIdx.x, xcd = bid % XCDS, row = bid / XCDS - crow;
+  const int njobs = batch.njobs, crow = batch.crow, wtot = batch.total, rtot = batch.rtotal;
+  const int nbase = batch.nbase;
+  const int fl = batch.first[lane0 < MAXJ ? lane0 : 0], rl = batch.rfirst[lane0 < MAXJ ? lane0 : 0];
+  const int bid = blockIdx.x, xcd = bid % XCDS, row = (bid - nbase) / XCDS;
   const bool crit = bid < njobs;
+  const int cb = bid - XCDS * crow;   // column workgroup index
+  const bool colwg = cb >= 0 && cb < rtot;
   const int g = (xcd * wtot) / XCDS + row;
-  const unsigned long long hit = __ballot(lane0 < njobs && fl <= g);
+  const int key = colwg ? rl : fl, want = colwg ? cb : g;
+  const unsigned long long hit = __ballot(lane0 < njobs && key <= want);
   const int jn = hit ? 63 - __clzll(hit) : 0;
-  const int base = __builtin_amdgcn_readlane(fl, jn);
-  // idle slots beside the critical workgroups, and the end of each range
-  if (!crit && (row < 0 || g >= ((xcd + 1) * wtot) / XCDS)) return;
+  const int base = __builtin_amdgcn_readlane(key, jn);
+  // idle slots beside the critical and the column workgroups, and the end
+  // of each range
+  if (!crit && !colwg && (bid < nbase || g >= ((xcd + 1) * wtot) / XCDS)) return;
   const int jb = crit ? bid : jn;
   const FJob<T> &J = batch.job[jb];
   in_sgprs(J);
-  const int local = crit ? critical_tile(J) : step_tile_of(J, g - base);
   // the critical workgroups bound the launch: their waves issue first on
   // the CUs they share with the others
-  if ((int)blockIdx.x < batch.njobs) __builtin_amdgcn_s_setprio(3);
+  if (crit) __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int qi = w >> 1, qj = w & 1;
   const size_t np2 = (size_t)J.np * J.np;
   const int nt = J.nt;
 
   if (J.k < 0) {  // initial pivot: inverse of T_k0[0,0], k0 the chain's first block
-    if (local == 0) {
+    if (crit) {
       if (J.dsym && J.dir > 0) block_symmetrizer(J);
       load_tile(J, J.dir > 0 ? 0 : J.m - 1, 0, (const T *)nullptr, 0, 0, L.Ar, L.Ai);
       __syncthreads();
@@ -913,13 +1079,20 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
     return;
   }
   const int k = J.k, p = J.p;
-  const int tj = local % nt, ti = local / nt;
   T *Gk = J.G + (size_t)k * 2 * np2;
   const T *src = (p == 0) ? nullptr : (((nt - p) % 2 == 0) ? Gk : J.W);
   T *dst = ((nt - 1 - p) % 2 == 0) ? Gk : J.W;
   const long step = (long)(J.dir > 0 ? k : J.m - 1 - k) * nt + p;   // of this chain
   const T *Pslot = J.Pinv + (size_t)(step & 1) * 2 * TB * TB;
   T *Pnext = J.Pinv + (size_t)((step + 1) & 1) * 2 * TB * TB;
+  if (colwg) {
+    column_step<T>(J, L, cb - base, src, dst, Pslot, (unsigned)(step + 1));
+    return;
+  }
+  const int local = crit ? critical_tile(J) : step_tile_of(J, g - base);
+  const int tj = local % nt, ti = local / nt;
+  // a tile off row and column p takes R[J] from its column workgroup
+  const bool with_r = !crit && uses_columns(J) && tj != p && ti != p;
 
   // every global read of this tile's update, issued together. P goes to the
   // B buffer while X[p,J] is in the A buffer (R = P X[p,J]), or to the A
@@ -927,25 +1100,73 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   // the B buffer once R has replaced X[p,J].
   typename Mfma<T>::v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
   const bool has_a = tj != p, has_b = ti != p;
+  // the reads of P and X[p,J]: not for a tile that takes R[J]
+  const bool own_r = !with_r;
   T (*Pbr)[TBP] = has_a ? L.Br : L.Ar;
   T (*Pbi)[TBP] = has_a ? L.Bi : L.Ai;
   constexpr int PER = TB * TB / NT;
   T br[PER], bi[PER];
   // symmetric jobs read X[p,J] (J < p) as tile (J,p), X[I,p] (I > p) as (p,I)
-  const bool mirA = p > 0 && J.dsym && tj < p, mirB = p > 0 && J.dsym && ti > p;
-  if (p > 0) {
+  const bool mirA = own_r && p > 0 && J.dsym && tj < p, mirB = p > 0 && J.dsym && ti > p;
+  if (p > 0 && with_r) {
+    // straight-line, so the reads issue together: the column's flag first
+    // (its answer comes back while the tile reads are in flight), X[I,p]
+    // (through the symmetry when I > p, with its symmetrizer entries read
+    // either way), the tile itself; then R[J] once the flag is up
+    const int a = tid >> 3, b0 = (tid & 7) * 4;
+    const unsigned tag = (unsigned)(step + 1);
+    const unsigned flag0 = ld_xcd(J.rflag + tj);
+    const size_t ib = mirB ? (size_t)(p * TB + a) * J.np + ti * TB + b0
+                           : (size_t)(ti * TB + a) * J.np + p * TB + b0;
+    ld_row4(src + ib, br);
+    ld_row4(src + np2 + ib, bi);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int ra = qi * 16 + Mfma<T>::crow(lane, t), rb = qj * 16 + (lane & 15);
+      const size_t idx = (size_t)(ti * TB + ra) * J.np + tj * TB + rb;
+      cr[t] = src[idx];
+      ci[t] = src[np2 + idx];
+    }
+    T er[PER], ei[PER];
+    const T dr = J.dsym[p * TB + a], di = J.dsym[J.np + p * TB + a];
+    ld_row4(J.dsym + 2 * J.np + ti * TB + b0, er);
+    ld_row4(J.dsym + 3 * J.np + ti * TB + b0, ei);
+    if (flag0 < tag) wave_wait_column(J, tj, tag, batch.cap);
+    const T *R = J.rbuf + (size_t)tj * 2 * TB * TB;
+    T rbr[PER], rbi[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      rbr[u] = ld_xcd(R + a * TB + b0 + u);
+      rbi[u] = ld_xcd(R + TB * TB + a * TB + b0 + u);
+    }
+    // X[I,p][b][a] = (D[32 p + a] / D[32 I + b]) X[p,I][a][b] when mirrored
+    T *Bflat_r = &L.Br[0][0], *Bflat_i = &L.Bi[0][0];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const T sr = dr * er[u] - di * ei[u], si = dr * ei[u] + di * er[u];
+      const T xr = mirB ? sr * br[u] - si * bi[u] : br[u];
+      const T xi = mirB ? sr * bi[u] + si * br[u] : bi[u];
+      const int at = mirB ? (b0 + u) * TBP + a : a * TBP + b0 + u;
+      Bflat_r[at] = xr;
+      Bflat_i[at] = xi;
+      L.Ar[a][b0 + u] = rbr[u];
+      L.Ai[a][b0 + u] = rbi[u];
+    }
+  } else if (p > 0) {
     // plain loads of the previous launch's output, all of them before any
     // use: thread t reads four consecutive elements of row t/8 of each tile
     // as one 16-byte access per plane
     T pr[PER], pim[PER], ar[PER], ai[PER];
     const int a = tid >> 3, b0 = (tid & 7) * 4;
-    ld_row4(Pslot + a * TB + b0, pr);
-    ld_row4(Pslot + TB * TB + a * TB + b0, pim);
+    if (own_r) {
+      ld_row4(Pslot + a * TB + b0, pr);
+      ld_row4(Pslot + TB * TB + a * TB + b0, pim);
+    }
     const size_t ia = mirA ? (size_t)(tj * TB + a) * J.np + p * TB + b0
                            : (size_t)(p * TB + a) * J.np + tj * TB + b0;
     const size_t ib = mirB ? (size_t)(p * TB + a) * J.np + ti * TB + b0
                            : (size_t)(ti * TB + a) * J.np + p * TB + b0;
-    if (has_a) {
+    if (has_a && own_r) {
       ld_row4(src + ia, ar);
       ld_row4(src + np2 + ia, ai);
     }
@@ -976,15 +1197,17 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int b = b0 + u;
-      Pbr[a][b] = pr[u];
-      Pbi[a][b] = pim[u];
+      if (own_r) {
+        Pbr[a][b] = pr[u];
+        Pbi[a][b] = pim[u];
+      }
       // sign * D[32 R + a] / D[32 C + b]
       const T sg = mirA ? T(-1) : T(1);
       const T rr = sg * (dr * er[u] - di * ei[u]), ri = sg * (dr * ei[u] + di * er[u]);
       if (mirA) {  // X[p,J][b][a] = -(D[32 J + a] / D[32 p + b]) X[J,p][a][b]
         L.Ar[b][a] = rr * ar[u] - ri * ai[u];
         L.Ai[b][a] = rr * ai[u] + ri * ar[u];
-      } else if (has_a) {
+      } else if (has_a && own_r) {
         L.Ar[a][b] = ar[u];
         L.Ai[a][b] = ai[u];
       }
@@ -996,11 +1219,12 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
     }
   } else {
     // p == 0: the tiles of S_k = T_k - l_k G_{k-1} u_{k-1}, formed on load
-    for (int e = tid; e < TB * TB; e += NT) {
-      Pbr[e / TB][e % TB] = Pslot[e];
-      Pbi[e / TB][e % TB] = Pslot[TB * TB + e];
-    }
-    if (has_a) load_tile(J, k, p, src, p, tj, L.Ar, L.Ai);   // X[p, J]
+    if (own_r)
+      for (int e = tid; e < TB * TB; e += NT) {
+        Pbr[e / TB][e % TB] = Pslot[e];
+        Pbi[e / TB][e % TB] = Pslot[TB * TB + e];
+      }
+    if (has_a && own_r) load_tile(J, k, p, src, p, tj, L.Ar, L.Ai);   // X[p, J]
     if (has_a && has_b) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
@@ -1048,6 +1272,23 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
       load_tile(J, k, p, src, ti, p, L.Br, L.Bi);
     }
   };
+  if (with_r) {
+    // R[J] from its column workgroup in the A buffer and X[I,p] in B: read
+    // with the tiles when p > 0, here at p = 0
+    if (p == 0) {
+      wait_column(J, tj, (unsigned)(step + 1), batch.cap);
+      const int a = tid >> 3, b0 = (tid & 7) * 4;
+      const T *R = J.rbuf + (size_t)tj * 2 * TB * TB;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        L.Ar[a][b0 + u] = ld_xcd(R + a * TB + b0 + u);
+        L.Ai[a][b0 + u] = ld_xcd(R + TB * TB + a * TB + b0 + u);
+      }
+      put_b();
+    }
+    __syncthreads();
+    cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Ar, L.Ai, qi, qj, lane, T(-1));
+  } else {
   if (!has_a && has_b) put_b();   // column p: P is in the A buffer
   __syncthreads();
   if (has_a) {  // R = P X[p, J]
@@ -1073,6 +1314,7 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
     }
   } else if (has_b) {  // -X[I,p] P (column p) or X[I,J] - X[I,p] R; the right factor is in the A buffer
     cgemm_quad<T>(cr, ci, L.Br, L.Bi, L.Ar, L.Ai, qi, qj, lane, T(-1));
+  }
   }
   // the tile goes out through LDS as whole rows (128-byte lines, 16-byte
   // stores) that bypass this XCD's L2: the next launch reads it from memory,
@@ -2501,16 +2743,23 @@ inline size_t elem_size(int precision) { return base_prec(precision) == FDFD_PRE
 
 struct FactorWork {
   void *W; void *Pinv; int *status;
+  unsigned *rflag;                             // [2 chains][MAX_TILES] column flags of the step kernel
   void *xcol; void *pinv_all; unsigned *sync;  // persistent-kernel exchange (W doubles as xrow)
   double *kscratch;                            // ping-pong partner of K (order 2n, complex FP64)
 };
+// The workspace header: the status word, then the column flags of both
+// chains; it is cleared as one block at the start of a factorization.
+constexpr int MAX_TILES = MAX_BLOCK_N / TB;
+constexpr size_t WORK_HEADER = 1024;
+static_assert(512 + 2 * MAX_TILES * 4 <= WORK_HEADER, "column flags fit the header");
 size_t persist_sync_words(int nt) { return 2 + (size_t)nt + 2 * (size_t)nt * nt; }
 FactorWork factor_work_layout(const fdfd_grid *g, int precision, void *work) {
   const size_t np = pad_n(g->ny), nt = np / TB, es = elem_size(precision);
   char *base = (char *)work;
   FactorWork fw;
   fw.status = (int *)base;
-  size_t off = 256;
+  fw.rflag = (unsigned *)(base + 512);
+  size_t off = WORK_HEADER;
   fw.Pinv = base + off;     off += align_up(2 * 2 * TB * TB * es, 256);
   fw.W = base + off;        off += align_up(2 * 2 * np * np * es, 256);  // also xrow (2 parities)
   fw.xcol = base + off;     off += align_up(2 * 2 * np * np * es, 256);
@@ -2521,7 +2770,7 @@ FactorWork factor_work_layout(const fdfd_grid *g, int precision, void *work) {
 }
 size_t factor_work_bytes(const fdfd_grid *g, int precision) {
   const size_t np = pad_n(g->ny), nt = np / TB, es = elem_size(precision), N = 2 * (size_t)g->ny;
-  return 256 + align_up(2 * 2 * TB * TB * es, 256) + 2 * align_up(2 * 2 * np * np * es, 256) +
+  return WORK_HEADER + align_up(2 * 2 * TB * TB * es, 256) + 2 * align_up(2 * 2 * np * np * es, 256) +
          align_up(2 * nt * 2 * TB * TB * es, 256) + align_up(4 * persist_sync_words((int)nt), 256) +
          align_up(16 * N * N, 256);
 }
@@ -2669,12 +2918,18 @@ int chain_jobs(const fdfd_grid *g, const double *op, void *factor, void *work, b
   memset(J, 0, sizeof(FJob<T>));
   J[0].op = op; J[0].G = (T *)factor; J[0].W = (T *)fw.W; J[0].Pinv = (T *)fw.Pinv; J[0].status = fw.status;
   J[0].dsym = sym ? (T *)fw.xcol : nullptr;
+  // the column buffers follow the symmetrizer in the persistent kernel's
+  // column area (4 np + 2 x 2 np TB elements of its 4 np^2)
+  J[0].rbuf = (T *)((char *)fw.xcol + align_up(4 * (size_t)np * sizeof(T), 256));
+  J[0].rflag = (sym && nt > 1) ? fw.rflag : nullptr;
   J[0].m = m; J[0].n = n; J[0].np = np; J[0].nt = nt;
   J[0].k = -1; J[0].p = 0; J[0].dir = 1; J[0].c = c;
   if (c == m - 1) return 1;
   J[1] = J[0];
   J[1].W = (T *)fw.W + 2 * (size_t)np * np;
   J[1].Pinv = (T *)fw.pinv_all;
+  J[1].rbuf = J[0].rbuf + 2 * (size_t)np * TB;
+  J[1].rflag = J[0].rflag ? fw.rflag + MAX_TILES : nullptr;
   J[1].dir = -1;
   return 2;
 }
@@ -2693,7 +2948,7 @@ inline void chain_at(FJob<T> &J, int st) {
 template <typename T, bool PIV>
 int factor_impl(const fdfd_grid *g, const double *op, void *factor, void *work, hipStream_t s) {
   FactorWork fw = factor_work_layout(g, sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64, work);
-  HIP_TRY(hipMemsetAsync(fw.status, 0, sizeof(int), s));
+  HIP_TRY(hipMemsetAsync(fw.status, 0, WORK_HEADER, s));
   FBatch<T> b;
   memset(&b, 0, sizeof b);
   const int nch = chain_jobs<T>(g, op, factor, work, g_step_sym.load() != 0, b.job);
@@ -2724,7 +2979,7 @@ int factor_batch_impl(const fdfd_grid *g, const double *const *ops, void *const 
   int nch = 1;
   for (int j = 0; j < njobs; ++j) {
     FactorWork fw = factor_work_layout(g, sizeof(T) == 8 ? FDFD_PREC_FP64 : FDFD_PREC_C64, works[j]);
-    HIP_TRY(hipMemsetAsync(fw.status, 0, sizeof(int), s));
+    HIP_TRY(hipMemsetAsync(fw.status, 0, WORK_HEADER, s));
     FJob<T> pair[2];
     nch = chain_jobs<T>(g, ops[j], factors[j], works[j], g_step_sym.load() != 0, pair);
     top[j] = pair[0];
@@ -3176,7 +3431,7 @@ void engine_loop(fdfd_engine *E) {
         auto e = E->pending.front();
         E->pending.pop_front();
         FactorWork fw = factor_work_layout(&e->g, e->precision, e->work);
-        if (hipMemsetAsync(fw.status, 0, sizeof(int), E->stream) != hipSuccess) e->rc = FDFD_ERR_HIP;
+        if (hipMemsetAsync(fw.status, 0, WORK_HEADER, E->stream) != hipSuccess) e->rc = FDFD_ERR_HIP;
         E->active.push_back(e);
       }
       if (E->active.empty()) {
@@ -3408,6 +3663,7 @@ int fdfd_factor_status(const fdfd_grid *g, int precision, void *work, hipStream_
   int st = 0;
   HIP_TRY(hipMemcpyAsync(&st, fw.status, sizeof(int), hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
+  if (st == 2) return fail(FDFD_ERR_NUMERIC, "a wait inside a block factorization step timed out");
   if (st) return fail(FDFD_ERR_NUMERIC, "zero or non-finite pivot in the block factorization");
   return FDFD_OK;
 }
