@@ -71,7 +71,7 @@ def parse():
                     help="CPU pool size (0: the host cores of this process, at most 16)")
     ap.add_argument("--latency-requests", type=int, default=160,
                     help="requests of the latency leg (arrivals below capacity)")
-    ap.add_argument("--latency-load", type=float, default=0.3,
+    ap.add_argument("--latency-load", type=float, default=0.25,
                     help="arrival rate of the latency leg as a fraction of the FP64 path's request rate")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 byte-counter passes")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)

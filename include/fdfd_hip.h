@@ -159,7 +159,9 @@ int fdfd_factor_wrap(const fdfd_grid *g, const double *op, void *factor, int pre
                      void *work, hipStream_t stream);
 
 /* Reads the numeric status the factor kernels left in work (waits for the
- * stream): FDFD_OK or FDFD_ERR_NUMERIC. */
+ * stream): FDFD_OK, or FDFD_ERR_NUMERIC for a zero or non-finite pivot or
+ * for a bounded wait inside a step launch that gave up (the message says
+ * which). */
 int fdfd_factor_status(const fdfd_grid *g, int precision, void *work, hipStream_t stream);
 
 /* One block sweep with the factor: x += T^-1 r (nrhs vectors), without the
