@@ -1042,9 +1042,14 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   // waited for one read per job; an unrolled scalar search kept the CU's
   // scalar unit busy.)
   const int lane0 = threadIdx.x & 63;
+  // a critical workgroup's job is job[blockIdx]: its fields are read with
+  // the header (clamped, so every workgroup reads inside the batch)
+  const FJob<T> Jc = batch.job[(int)blockIdx.x < MAXJ ? (int)blockIdx.x : MAXJ - 1];
   const int njobs = batch.njobs, crow = batch.crow, wtot = batch.total, rtot = batch.rtotal;
   const int nbase = batch.nbase;
   const int fl = batch.first[lane0 < MAXJ ? lane0 : 0], rl = batch.rfirst[lane0 < MAXJ ? lane0 : 0];
+  in_sgprs(Jc);
+  asm volatile("" ::"s"(njobs), "s"(crow), "s"(wtot), "s"(rtot), "s"(nbase));
   const int bid = blockIdx.x, xcd = bid % XCDS, row = (bid - nbase) / XCDS;
   const bool crit = bid < njobs;
   const int cb = bid - XCDS * crow;   // column workgroup index
@@ -1057,9 +1062,11 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   // idle slots beside the critical and the column workgroups, and the end
   // of each range
   if (!crit && !colwg && (bid < nbase || g >= ((xcd + 1) * wtot) / XCDS)) return;
-  const int jb = crit ? bid : jn;
-  const FJob<T> &J = batch.job[jb];
-  in_sgprs(J);
+  FJob<T> Jv;
+  if (crit) Jv = Jc;
+  else Jv = batch.job[jn];
+  in_sgprs(Jv);
+  const FJob<T> &J = Jv;
   // the critical workgroups bound the launch: their waves issue first on
   // the CUs they share with the others
   if (crit) __builtin_amdgcn_s_setprio(3);
