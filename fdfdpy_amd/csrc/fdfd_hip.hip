@@ -890,21 +890,29 @@ __device__ inline void schur_next_00(const FJob<T> &J, int k, LT &L) {
 // D over one block from the couplings of block 0: D_{y+1} / D_y =
 // A[y, y+1] / A[y+1, y] (the same ratio in every block); 1 on the padding.
 template <typename T>
-__device__ void block_symmetrizer(const FJob<T> &J) {
+__device__ void block_symmetrizer(const double *op, int m, int n, int np, T *dsym) {
   if (threadIdx.x != 0) return;
-  const int n = J.n, np = J.np;
-  const size_t M = (size_t)J.m * n;
+  const size_t M = (size_t)m * n;
   cd d = {1.0, 0.0};
   for (int y = 0; y < np; ++y) {
     if (y >= n) d = {1.0, 0.0};
     const cd di = cinv(d);
-    J.dsym[y] = (T)d.r;
-    J.dsym[np + y] = (T)d.i;
-    J.dsym[2 * np + y] = (T)di.r;
-    J.dsym[3 * np + y] = (T)di.i;
+    dsym[y] = (T)d.r;
+    dsym[np + y] = (T)d.i;
+    dsym[2 * np + y] = (T)di.r;
+    dsym[3 * np + y] = (T)di.i;
     if (y + 1 < n)
-      d = cmul(d, cmul(ld_c(J.op, FDFD_OP_YP * M + y), cinv(ld_c(J.op, FDFD_OP_YM * M + y + 1))));
+      d = cmul(d, cmul(ld_c(op, FDFD_OP_YP * M + y), cinv(ld_c(op, FDFD_OP_YM * M + y + 1))));
   }
+}
+template <typename T>
+__device__ void block_symmetrizer(const FJob<T> &J) {
+  block_symmetrizer(J.op, J.m, J.n, J.np, J.dsym);
+}
+// the same D for the corner chains' products, on their own stream
+template <typename T>
+__global__ void k_block_symmetrizer(const double *op, int m, int n, int np, T *dsym) {
+  block_symmetrizer(op, m, n, np, dsym);
 }
 
 // Thread 0 waits until column J's flag holds the step's tag (or the job's
@@ -2251,6 +2259,10 @@ template <typename T> struct GemmJob {
   T *C;
   const double *rs, *cs;
   T sign;
+  // [4][np] D, 1/D of the block symmetrizer: C is D-symmetric
+  // (C[y][x] = (D[x] / D[y]) C[x][y]); only its tiles I <= J are computed and
+  // the others written as their mirror images. Null: every tile.
+  const T *dsym;
 };
 template <typename T> struct GemmBatch {
   int njobs;
@@ -2307,9 +2319,12 @@ template <typename T>
 __global__ void __launch_bounds__(GT) k_cgemm(GemmBatch<T> batch, int n, int np) {
   __shared__ T Ar[2][TB][TBP], Ai[2][TB][TBP], Br[2][TB][TBP], Bi[2][TB][TBP];
   const int nt = np / TB;
-  const int jb = blockIdx.x / (nt * nt), t = blockIdx.x % (nt * nt);
-  const int ti = t / nt, tj = t % nt;
+  // both jobs of a launch compute the same tiles: all of them, or I <= J
+  const int per = batch.job[0].dsym ? nt * (nt + 1) / 2 : nt * nt;
+  const int jb = blockIdx.x / per, t = blockIdx.x % per;
   const GemmJob<T> J = jb == 0 ? batch.job[0] : batch.job[1];
+  const int tu = J.dsym ? upper_tile(nt, t) : t;
+  const int ti = tu / nt, tj = tu % nt;
   const int grp = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int lane = tid & 63, wv = tid >> 6, qi = wv >> 1, qj = wv & 1;
   const size_t np2 = (size_t)np * np;
@@ -2336,6 +2351,22 @@ __global__ void __launch_bounds__(GT) k_cgemm(GemmBatch<T> batch, int n, int np)
   if (grp == 1) acc_to_lds<T>(cr, ci, Ar[0], Ai[0], qi, qj, lane);
   __syncthreads();
   if (grp == 1) return;
+  const bool mirror = J.dsym && ti != tj;
+  // the mirror tile (J, I): this lane's element (a, b) goes to row
+  // y = 32 J + b, column x = 32 I + a, scaled by D[x] / D[y]
+  T dxr[4] = {0, 0, 0, 0}, dxi[4] = {0, 0, 0, 0}, eyr = 0, eyi = 0;
+  if (mirror) {
+    const int y = tj * TB + qj * 16 + (lane & 15);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int x = ti * TB + qi * 16 + Mfma<T>::crow(lane, q);
+      dxr[q] = J.dsym[x];
+      dxi[q] = J.dsym[np + x];
+    }
+    eyr = J.dsym[2 * np + y];
+    eyi = J.dsym[3 * np + y];
+  }
+  T mr[4], mi[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int a = qi * 16 + Mfma<T>::crow(lane, q), b = qj * 16 + (lane & 15);
@@ -2345,6 +2376,18 @@ __global__ void __launch_bounds__(GT) k_cgemm(GemmBatch<T> batch, int n, int np)
     if (J.D) { vr += J.D[idx]; vi += J.D[np2 + idx]; }
     J.C[idx] = vr;
     J.C[np2 + idx] = vi;
+    const T rr = dxr[q] * eyr - dxi[q] * eyi, ri = dxr[q] * eyi + dxi[q] * eyr;
+    mr[q] = rr * vr - ri * vi;
+    mi[q] = rr * vi + ri * vr;
+  }
+  if (mirror) {
+    const int y = tj * TB + qj * 16 + (lane & 15);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const size_t midx = (size_t)y * np + ti * TB + qi * 16 + Mfma<T>::crow(lane, q);
+      J.C[midx] = mr[q];
+      J.C[np2 + midx] = mi[q];
+    }
   }
 }
 
@@ -3094,6 +3137,11 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
   const T *sig_prev = G + (size_t)L * 2 * np2, *z_prev = exact ? Glast : nullptr;
   T *SigB = (T *)fw.xcol, *QB = (T *)fw.xcol + 2 * np2;
   const T *sigb_prev = G + (size_t)(m - 1 - L) * 2 * np2;
+  // Sigma_k and Sigma'_k are D-symmetric like G_k: the products that form
+  // them compute their upper tiles only. D goes to the pivot area of the
+  // workspace (free once the block factorization is done).
+  T *dsym = (T *)fw.pinv_all;
+  hipLaunchKernelGGL(k_block_symmetrizer<T>, dim3(1), dim3(64), 0, s, op, m, n, np, dsym);
   int zi = 0;
   for (int i = 0; i < L; ++i) {
     const int k = L - 1 - i, kb = m - L + i;   // top and bottom chain blocks
@@ -3117,9 +3165,9 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
     GemmBatch<T> b2;
     memset(&b2, 0, sizeof b2);
     b2.njobs = 1;
-    b2.job[0] = GemmJob<T>{Q, Gk, Gk, Sig, nullptr, nullptr, T(1)};
-    if (twisted) b2.job[b2.njobs++] = GemmJob<T>{QB, Gb, Gb, SigB, nullptr, nullptr, T(1)};
-    hipLaunchKernelGGL(k_cgemm<T>, dim3(b2.njobs * nt * nt), dim3(GT), 0, s, b2, n, np);
+    b2.job[0] = GemmJob<T>{Q, Gk, Gk, Sig, nullptr, nullptr, T(1), dsym};
+    if (twisted) b2.job[b2.njobs++] = GemmJob<T>{QB, Gb, Gb, SigB, nullptr, nullptr, T(1), dsym};
+    hipLaunchKernelGGL(k_cgemm<T>, dim3(b2.njobs * nt * (nt + 1) / 2), dim3(GT), 0, s, b2, n, np);
     sig_prev = Sig;
     sigb_prev = SigB;
     if (exact) {
