@@ -575,16 +575,24 @@ __device__ inline void load_tile(const FJob<T> &J, int k, int p, const T *src, i
   }
 }
 
-// c += sign * A[quadrant rows, :] * B[:, quadrant cols] (complex, on MFMA)
+// c += sign * A[quadrant rows, :] * B[:, quadrant cols] (complex, on MFMA).
+// Lane group g = lane / 16 supplies index k = 2 s + g / 2 + 16 (g % 2) of
+// the inner dimension at MFMA s (s = 0..7; every k once): within each half
+// wave the rows r + 16 (g % 2) of A and the columns of B then fall in 32
+// distinct LDS banks under the 33-word row pitch (consecutive k, as k = 4 s
+// + g, put rows r and r + 1 of the two groups on one bank).
 template <typename T>
 __device__ inline void cgemm_quad(typename Mfma<T>::v4 &cr, typename Mfma<T>::v4 &ci,
                                   const T (*Ar)[TBP], const T (*Ai)[TBP], const T (*Br)[TBP],
                                   const T (*Bi)[TBP], int qi, int qj, int lane, T sign) {
+  static_assert(TB == 32 && TBP == 33, "the inner-index order assumes 32-wide tiles, 33-word rows");
   const int r = lane & 15, kq = lane >> 4;
+  const int k0 = (kq >> 1) + 16 * (kq & 1);
 #pragma unroll
-  for (int kk = 0; kk < TB; kk += 4) {
-    const T ar = sign * Ar[qi * 16 + r][kk + kq], ai = sign * Ai[qi * 16 + r][kk + kq];
-    const T br = Br[kk + kq][qj * 16 + r], bi = Bi[kk + kq][qj * 16 + r];
+  for (int s = 0; s < TB / 4; ++s) {
+    const int k = 2 * s + k0;
+    const T ar = sign * Ar[qi * 16 + r][k], ai = sign * Ai[qi * 16 + r][k];
+    const T br = Br[k][qj * 16 + r], bi = Bi[k][qj * 16 + r];
     cr = Mfma<T>::mac(ar, br, cr);
     cr = Mfma<T>::mac(-ai, bi, cr);
     ci = Mfma<T>::mac(ar, bi, ci);
