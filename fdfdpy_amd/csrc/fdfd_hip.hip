@@ -959,6 +959,23 @@ __device__ inline void wave_wait_column(const FJob<T> &J, int col, unsigned tag,
   }
 }
 
+// Row a, columns b0..b0+3 of R[col] once its flag is up, as plain 16-byte
+// loads. They may be served by this XCD's L2: a line of R[col] reaches an
+// L2 only through a read made after the flag was seen raised, when the
+// column workgroup's write-through stores have completed (caches hold no
+// line of the buffer from an earlier launch), so the first reader on an XCD
+// fetches it from memory and the others hit. The empty asm keeps the
+// compiler from moving the loads above the flag reads; it adds no wait on
+// the loads already in flight.
+template <typename T>
+__device__ inline void ld_column(const FJob<T> &J, int col, int a, int b0, T (&vr)[TB * TB / NT],
+                                 T (&vi)[TB * TB / NT]) {
+  asm volatile("" ::: "memory");
+  const T *R = J.rbuf + (size_t)col * 2 * TB * TB;
+  ld_row4(R + a * TB + b0, vr);
+  ld_row4(R + TB * TB + a * TB + b0, vi);
+}
+
 // The column workgroup r of a job: R[J] = P X[p,J] for column J = r (r < p)
 // or r + 1, into the job's column buffer (agent-coherent stores, then the
 // flag); right of the diagonal R[J] is also the step's result for tile
@@ -1155,13 +1172,8 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
     ld_row4(J.dsym + 2 * J.np + ti * TB + b0, er);
     ld_row4(J.dsym + 3 * J.np + ti * TB + b0, ei);
     if (flag0 < tag) wave_wait_column(J, tj, tag, batch.cap);
-    const T *R = J.rbuf + (size_t)tj * 2 * TB * TB;
     T rbr[PER], rbi[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      rbr[u] = ld_xcd(R + a * TB + b0 + u);
-      rbi[u] = ld_xcd(R + TB * TB + a * TB + b0 + u);
-    }
+    ld_column(J, tj, a, b0, rbr, rbi);
     // X[I,p][b][a] = (D[32 p + a] / D[32 I + b]) X[p,I][a][b] when mirrored
     T *Bflat_r = &L.Br[0][0], *Bflat_i = &L.Bi[0][0];
 #pragma unroll
@@ -1301,11 +1313,12 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
     if (p == 0) {
       wait_column(J, tj, (unsigned)(step + 1), batch.cap);
       const int a = tid >> 3, b0 = (tid & 7) * 4;
-      const T *R = J.rbuf + (size_t)tj * 2 * TB * TB;
+      T rbr[PER], rbi[PER];
+      ld_column(J, tj, a, b0, rbr, rbi);
 #pragma unroll
       for (int u = 0; u < PER; ++u) {
-        L.Ar[a][b0 + u] = ld_xcd(R + a * TB + b0 + u);
-        L.Ai[a][b0 + u] = ld_xcd(R + TB * TB + a * TB + b0 + u);
+        L.Ar[a][b0 + u] = rbr[u];
+        L.Ai[a][b0 + u] = rbi[u];
       }
       put_b();
     }
