@@ -16,7 +16,7 @@ value) and for the FP64-factor path on the same trace.
 Two loads: the throughput leg offers the stream faster than it is served
 (Poisson arrivals at --rate), so solves/s is the saturated rate and its
 latencies include queueing; the latency leg replays the first
---latency-requests requests at 0.3 x the FP64 path's saturated request rate,
+--latency-requests requests at --latency-load (0.25) x the FP64 path's saturated request rate,
 below both paths' capacity at that load (the factorizations batch less when
 fewer requests are in flight, so capacity falls with the load), and its
 p50/p99 are the headline latencies.
@@ -525,6 +525,9 @@ def config_timings():
 def run_stream(reqs, precision, slots, tol, engines=2, keep=False):
     import torch
     from fdfdpy_amd.serving import ServingEngine, summarize
+    # the previous leg's blocks go back to the device before this leg sizes
+    # its memory budget (outside the timed region)
+    torch.cuda.empty_cache()
     eng = ServingEngine("cuda:0", precision=precision, slots=slots, tol=tol, engines=engines, keep_fields=keep)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -536,6 +539,9 @@ def run_stream(reqs, precision, slots, tol, engines=2, keep=False):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     summ = summarize(res, wall)
+    errs = sorted({r.error for r in res if r.error})
+    if errs:
+        log(f"  {precision}: {summ['failed']} failed requests; errors: {errs[:4]}")
     if eng.factor_engines:
         # batching of the factorizations: job-steps per shared step launch
         steps = sum(r.problem.shape[0] * (-(-r.problem.shape[1] // 32)) + 1 for r in reqs)
@@ -638,7 +644,7 @@ def main():
     if args.stream_only:
         print(json.dumps({"stream_c64": mixed, "stream_fp64": fp64}), flush=True)
         return
-    # latency leg: a prefix of the stream at 0.3 x the FP64 path's saturated request rate
+    # latency leg: a prefix of the stream at latency_load x the FP64 path's saturated request rate
     cap = (fp64 or mixed)["requests_per_s"]
     lat_rate = args.latency_load * cap
     k = min(args.latency_requests, n_req)

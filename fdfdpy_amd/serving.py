@@ -111,7 +111,8 @@ class ServingEngine:
     precision: 'c64' (complex64 factor + FP64 refinement, FP64 fallback) or
     'fp64'. slots: requests in flight at once (one host thread and one HIP
     stream each). memory_budget: device bytes for in-flight requests
-    (default: 85% of the free memory at construction). cache_bytes: device
+    (default: 85% of the memory free at construction, counting the
+    blocks torch's caching allocator holds unused). cache_bytes: device
     bytes for factors kept after their request completes. batch_factors:
     the factorizations of in-flight requests advance together in shared
     Gauss-Jordan step launches (the native engine) instead of one launch
@@ -127,8 +128,12 @@ class ServingEngine:
         self.precision = precision
         self.slots = slots
         if memory_budget is None:
+            # free device memory plus the blocks torch's caching allocator
+            # holds unused (earlier work on this device may have left most of
+            # the memory there, where mem_get_info does not count it)
             free, _ = torch.cuda.mem_get_info(self.device)
-            memory_budget = int(0.85 * free)
+            cached = torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
+            memory_budget = int(0.85 * (free + max(cached, 0)))
         self.memory_budget = memory_budget
         self.tol = tol
         self.max_iter = max_iter
