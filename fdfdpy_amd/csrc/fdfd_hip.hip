@@ -899,18 +899,33 @@ __device__ inline void schur_next_00(const FJob<T> &J, int k, LT &L) {
 // A[y, y+1] / A[y+1, y] (the same ratio in every block); 1 on the padding.
 template <typename T>
 __device__ void block_symmetrizer(const double *op, int m, int n, int np, T *dsym) {
-  if (threadIdx.x != 0) return;
+  // D_0 = 1, D_y = D_{y-1} A[y-1,y] / A[y,y-1] for y < n, 1 beyond: a
+  // product scan by the first wave, 64 entries per round (lane l takes the
+  // ratio into y = y0 + l; a shuffle scan forms the products within the
+  // round, the last lane's product carries to the next)
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
   const size_t M = (size_t)m * n;
-  cd d = {1.0, 0.0};
-  for (int y = 0; y < np; ++y) {
-    if (y >= n) d = {1.0, 0.0};
-    const cd di = cinv(d);
-    dsym[y] = (T)d.r;
-    dsym[np + y] = (T)d.i;
-    dsym[2 * np + y] = (T)di.r;
-    dsym[3 * np + y] = (T)di.i;
-    if (y + 1 < n)
-      d = cmul(d, cmul(ld_c(op, FDFD_OP_YP * M + y), cinv(ld_c(op, FDFD_OP_YM * M + y + 1))));
+  cd carry = {1.0, 0.0};
+  for (int y0 = 0; y0 < np; y0 += 64) {
+    const int y = y0 + lane;
+    cd r = {1.0, 0.0};
+    if (y >= 1 && y < n) r = cmul(ld_c(op, FDFD_OP_YP * M + y - 1), cinv(ld_c(op, FDFD_OP_YM * M + y)));
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const cd o = {__shfl_up(r.r, off), __shfl_up(r.i, off)};
+      if (lane >= off) r = cmul(o, r);
+    }
+    const cd pd = cmul(carry, r);
+    carry = {__shfl(pd.r, 63), __shfl(pd.i, 63)};
+    const cd d = y < n ? pd : cd{1.0, 0.0};
+    if (y < np) {
+      const cd di = cinv(d);
+      dsym[y] = (T)d.r;
+      dsym[np + y] = (T)d.i;
+      dsym[2 * np + y] = (T)di.r;
+      dsym[3 * np + y] = (T)di.i;
+    }
   }
 }
 template <typename T>
