@@ -458,18 +458,17 @@ __device__ inline void ld_row4(const double *p, double (&v)[4]) {
   v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
 }
 
-// Four consecutive elements from LDS to global memory as 16-byte
-// non-temporal stores.
-__device__ inline void store4_nt(float *p, const float *v) {
+// Four consecutive elements to global memory as 16-byte stores. Plain
+// (write-back) stores: every reader of these is a later launch, and with
+// non-temporal stores saturated step launches took about 10% longer.
+__device__ inline void store4(float *p, const float *v) {
   typedef float f4 __attribute__((ext_vector_type(4)));
-  const f4 x = {v[0], v[1], v[2], v[3]};
-  __builtin_nontemporal_store(x, reinterpret_cast<f4 *>(p));
+  *reinterpret_cast<f4 *>(p) = f4{v[0], v[1], v[2], v[3]};
 }
-__device__ inline void store4_nt(double *p, const double *v) {
+__device__ inline void store4(double *p, const double *v) {
   typedef double d2 __attribute__((ext_vector_type(2)));
-  const d2 x = {v[0], v[1]}, y = {v[2], v[3]};
-  __builtin_nontemporal_store(x, reinterpret_cast<d2 *>(p));
-  __builtin_nontemporal_store(y, reinterpret_cast<d2 *>(p + 2));
+  reinterpret_cast<d2 *>(p)[0] = d2{v[0], v[1]};
+  reinterpret_cast<d2 *>(p)[1] = d2{v[2], v[3]};
 }
 
 // Element (a, b) of the working matrix of block k before step p. For p == 0
@@ -851,10 +850,10 @@ __device__ inline void tile_inverse(LT &L, int *status) {
 
 template <typename T, typename LT>
 __device__ inline void store_pinv(const LT &L, T *slot) {
-  // whole rows, 16-byte non-temporal stores (read by the next launch)
+  // whole rows, 16-byte stores (read by the next launch)
   const int i = threadIdx.x >> 3, c0 = (threadIdx.x & 7) * 4;
-  store4_nt(slot + i * TB + c0, &L.Ar[i][c0]);
-  store4_nt(slot + TB * TB + i * TB + c0, &L.Ai[i][c0]);
+  store4(slot + i * TB + c0, &L.Ar[i][c0]);
+  store4(slot + TB * TB + i * TB + c0, &L.Ai[i][c0]);
 }
 
 // The [0,0] tile of the chain's next block kn = k + dir from G_k[0,0] held
@@ -1059,8 +1058,8 @@ __device__ void column_step(const FJob<T> &J, LdsStep<T> &L, int r, const T *src
     }
     __syncthreads();
     const size_t idx = (size_t)(p * TB + a) * J.np + col * TB + b0;
-    store4_nt(dst + idx, &L.Ar[a][b0]);
-    store4_nt(dst + np2 + idx, &L.Ai[a][b0]);
+    store4(dst + idx, &L.Ar[a][b0]);
+    store4(dst + np2 + idx, &L.Ai[a][b0]);
   }
   stores_complete();
   __syncthreads();
@@ -1368,8 +1367,7 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   }
   }
   // the tile goes out through LDS as whole rows (128-byte lines, 16-byte
-  // stores) that bypass this XCD's L2: the next launch reads it from memory,
-  // possibly on another XCD
+  // stores); the next launch reads it, possibly on another XCD
   __syncthreads();  // all reads of Ar/Ai, Pr/Pi, Br/Bi are done
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -1385,8 +1383,8 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   if (!next_pivot) {
     const int i = tid >> 3, c0 = (tid & 7) * 4;
     const size_t idx = (size_t)(ti * TB + i) * J.np + tj * TB + c0;
-    store4_nt(dst + idx, &L.Ar[i][c0]);
-    store4_nt(dst + np2 + idx, &L.Ai[i][c0]);
+    store4(dst + idx, &L.Ar[i][c0]);
+    store4(dst + np2 + idx, &L.Ai[i][c0]);
     if (J.dsym && p == nt - 1 && ti != tj) {
       // G_k is symmetric under D: G_k[J,I][i][c] = (D[32 I + c] / D[32 J + i]) G_k[I,J][c][i]
       T mr[4], mi[4];
@@ -1400,8 +1398,8 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
         mi[q] = rr * vi + ri * vr;
       }
       const size_t mdx = (size_t)(tj * TB + i) * J.np + ti * TB + c0;
-      store4_nt(dst + mdx, mr);
-      store4_nt(dst + np2 + mdx, mi);
+      store4(dst + mdx, mr);
+      store4(dst + np2 + mdx, mi);
     }
   }
   if (next_pivot || next_block) {
