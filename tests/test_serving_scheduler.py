@@ -89,3 +89,16 @@ def test_factor_cache_lru():
     assert c.get("b") is None and c.get("a") == "A" and c.get("c") == "C"
     c.put("d", "D", 200)           # larger than the cache: not kept
     assert c.get("d") is None
+
+
+def test_default_budget_counts_the_caching_allocators_unused_blocks(monkeypatch):
+    # 10 GiB free on the device and 30 GiB held unused by torch's caching
+    # allocator (reserved 50, allocated 20): the budget is 85% of 40 GiB, not
+    # of the 10 GiB mem_get_info reports
+    import torch
+    gib = 1 << 30
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda device=None: (10 * gib, 288 * gib))
+    monkeypatch.setattr(torch.cuda, "memory_reserved", lambda device=None: 50 * gib)
+    monkeypatch.setattr(torch.cuda, "memory_allocated", lambda device=None: 20 * gib)
+    eng = ServingEngine("cpu", slots=2, serve_fn=FakeDevice())
+    assert eng.memory_budget == int(0.85 * 40 * gib)
