@@ -3185,15 +3185,15 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
     b1.njobs = 1;
     // Q = G_k (XP_k Sigma_{k+1} XM_{k+1})
     b1.job[0] = GemmJob<T>{Gk, sig_prev, nullptr, Q, op + 2 * (FDFD_OP_XP * M + (size_t)k * n),
-                           op + 2 * (FDFD_OP_XM * M + (size_t)(k + 1) * n), T(1)};
+                           op + 2 * (FDFD_OP_XM * M + (size_t)(k + 1) * n), T(1), nullptr};
     // Z_k = -G_k (XP_k Z_{k+1})
     if (exact)
       b1.job[b1.njobs++] = GemmJob<T>{Gk, z_prev, nullptr, Z[zi], op + 2 * (FDFD_OP_XP * M + (size_t)k * n),
-                                      nullptr, T(-1)};
+                                      nullptr, T(-1), nullptr};
     // Q' = G_kb (XM_kb Sigma'_{kb-1} XP_{kb-1})
     if (twisted)
       b1.job[b1.njobs++] = GemmJob<T>{Gb, sigb_prev, nullptr, QB, op + 2 * (FDFD_OP_XM * M + (size_t)kb * n),
-                                      op + 2 * (FDFD_OP_XP * M + (size_t)(kb - 1) * n), T(1)};
+                                      op + 2 * (FDFD_OP_XP * M + (size_t)(kb - 1) * n), T(1), nullptr};
     hipLaunchKernelGGL(k_cgemm<T>, dim3(b1.njobs * nt * nt), dim3(GT), 0, s, b1, n, np);
     // Sigma_k = G_k + Q G_k (and Sigma'_kb = G_kb + Q' G_kb)
     GemmBatch<T> b2;
@@ -3226,7 +3226,7 @@ int wrap_setup(const fdfd_grid *g, const double *op, void *factor, void *work, b
     memset(&b3, 0, sizeof b3);
     b3.njobs = 1;
     b3.job[0] = GemmJob<T>{sig0, sigl, nullptr, Q, op + 2 * (FDFD_OP_XM * M),
-                           op + 2 * (FDFD_OP_XP * M + (size_t)(m - 1) * n), T(-1)};
+                           op + 2 * (FDFD_OP_XP * M + (size_t)(m - 1) * n), T(-1), nullptr};
     hipLaunchKernelGGL(k_cgemm<T>, dim3(nt * nt), dim3(GT), 0, s, b3, n, np);
     const bool tiled = wrap_sinv_tiled(g, prec);
     if (tiled) {
