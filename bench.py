@@ -525,9 +525,9 @@ def config_timings():
 def run_stream(reqs, precision, slots, tol, engines=2, keep=False):
     import torch
     from fdfdpy_amd.serving import ServingEngine, summarize
-    # the previous leg's blocks go back to the device before this leg sizes
-    # its memory budget (outside the timed region)
-    torch.cuda.empty_cache()
+    # no empty_cache() here: the engine's budget counts the blocks torch's
+    # allocator holds, and a leg that starts from an empty cache allocates
+    # every request's buffers anew, which stalled whole runs for seconds
     eng = ServingEngine("cuda:0", precision=precision, slots=slots, tol=tol, engines=engines, keep_fields=keep)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -102,6 +102,9 @@ def lib():
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
                 "(the HIP library is required; there is no CPU fallback)")
+        # torch first: the library then binds to the HIP runtime torch
+        # already holds, so both work on the same device contexts
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
