@@ -522,12 +522,18 @@ def config_timings():
     return out
 
 
-def run_stream(reqs, precision, slots, tol, engines=2, keep=False):
+def run_stream(reqs, precision, slots, tol, engines=2, keep=False, fresh_cache=False):
     import torch
     from fdfdpy_amd.serving import ServingEngine, summarize
-    # no empty_cache() here: the engine's budget counts the blocks torch's
-    # allocator holds, and a leg that starts from an empty cache allocates
-    # every request's buffers anew, which stalled whole runs for seconds
+    # fresh_cache: the allocator's cached blocks go back to the device first
+    # (outside the timed region). The latency legs take it: few requests are
+    # in flight there, and blocks cut for the other precision's requests made
+    # the allocator free and reallocate memory under load, which stalled the
+    # FP64 leg for seconds. The throughput legs start from the warm cache: a
+    # leg that starts empty allocates every request's buffers anew under full
+    # load, which stalled about one run in three at its start.
+    if fresh_cache:
+        torch.cuda.empty_cache()
     eng = ServingEngine("cuda:0", precision=precision, slots=slots, tol=tol, engines=engines, keep_fields=keep)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -653,11 +659,11 @@ def main():
     larr = np.cumsum(lgaps) - lgaps[0]
     lreqs = [Request(i, problems[i], float(larr[i])) for i in range(k)]
     log(f"latency leg: {k} requests at {lat_rate:.2f}/s")
-    lat_m, _ = run_stream(lreqs, "c64", args.slots, args.tol, args.engines)
+    lat_m, _ = run_stream(lreqs, "c64", args.slots, args.tol, args.engines, fresh_cache=True)
     log(f"latency mixed: {json.dumps(lat_m)}")
     lat_f = None
     if not args.no_fp64:
-        lat_f, _ = run_stream(lreqs, "fp64", args.slots, args.tol, args.engines)
+        lat_f, _ = run_stream(lreqs, "fp64", args.slots, args.tol, args.engines, fresh_cache=True)
         log(f"latency fp64: {json.dumps(lat_f)}")
     roof = roofline_leg(traffic=traffic)
     log(f"roofline: {json.dumps(roof)}")
