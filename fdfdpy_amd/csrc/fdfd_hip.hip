@@ -899,9 +899,9 @@ __device__ inline void schur_next_00(const FJob<T> &J, int k, LT &L) {
 template <typename T>
 __device__ void block_symmetrizer(const double *op, int m, int n, int np, T *dsym) {
   // D_0 = 1, D_y = D_{y-1} A[y-1,y] / A[y,y-1] for y < n, 1 beyond: a
-  // product scan by the first wave, 64 entries per round (lane l takes the
+  // product scan by the first wave, 64 entries per pass (lane l takes the
   // ratio into y = y0 + l; a shuffle scan forms the products within the
-  // round, the last lane's product carries to the next)
+  // pass, the last lane's product carries to the next)
   if (threadIdx.x >= 64) return;
   const int lane = threadIdx.x;
   const size_t M = (size_t)m * n;
