@@ -47,7 +47,7 @@ sys.path.insert(0, ROOT)
 FP32_MFMA_PEAK_TFLOPS = 157.3
 FP64_MFMA_PEAK_TFLOPS = 78.6
 ROOF_N = 512          # grid of the roofline leg's factorizations
-ROOF_JOBS = 10        # factorizations per step launch: 20 chains (the stream's launches carry 19.6-21.6)
+ROOF_JOBS = 10        # factorizations per step launch: 20 chains (the stream's launches carry 18-22)
 COPY_BYTES = 256 << 20
 
 
