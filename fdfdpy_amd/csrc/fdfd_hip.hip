@@ -1114,9 +1114,10 @@ __global__ void __launch_bounds__(NT, 6) k_gj_step(FBatch<T> batch) {
   else Jv = batch.job[jn];
   in_sgprs(Jv);
   const FJob<T> &J = Jv;
-  // the critical workgroups bound the launch: their waves issue first on
+  // the critical workgroups bound the launch, and the column workgroups
+  // gate the tile workgroups of their columns: their waves issue first on
   // the CUs they share with the others
-  if (crit) __builtin_amdgcn_s_setprio(3);
+  if (crit || colwg) __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int qi = w >> 1, qj = w & 1;
   const size_t np2 = (size_t)J.np * J.np;
