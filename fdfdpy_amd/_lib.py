@@ -103,7 +103,8 @@ def lib():
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
                 "(the HIP library is required; there is no CPU fallback)")
         # torch first: the library then binds to the HIP runtime torch
-        # already holds, so both work on the same device contexts
+        # already holds (loaded before torch, the library's engine could not
+        # create its stream on the device)
         import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
